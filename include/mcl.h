@@ -1,0 +1,211 @@
+/*
+ * mcl.h — C-ABI of libmcl_hip.so, the MI355X (gfx950) implementation of the
+ * Mantle_Convection_Lab per-time-step hot path.
+ *
+ * The reference calls its hot path as K&R `void f()` functions that read and
+ * write the global struct G (function_def.h:73,89,91,102,177).  A reference
+ * build that uses this library keeps those signatures and forwards each call to
+ * the mcl_* entry point with the same name:
+ *
+ *   buoyancy_force()               Buoyancy.c:3        -> mcl_buoyancy_force
+ *   viscosity()                    Viscosity.c:3       -> mcl_viscosity
+ *   velocity_pressure_KGF_matrix() Build_matrix.c:4    -> mcl_velocity_pressure_KGF_matrix
+ *   solve_velocity_pressure()      Solver.c:3          -> mcl_solve_velocity_pressure
+ *   update_temperature()           Update_temperature.c:3 -> mcl_update_temperature
+ *
+ * Mesh arrays are passed once per mesh (first step, or every step when AMR=1,
+ * Main.c:33-52) through mcl_mesh_upload as borrowed host pointers in the
+ * reference's own layouts; the library copies them and keeps no host pointer.
+ * Fields stay resident in device memory between calls; mcl_set_field /
+ * mcl_get_field move them between the caller's G arrays and the device.
+ *
+ * Every entry point returns 0 on success or a positive MCL_E_* code; the
+ * message of the last failure is available from mcl_last_error().  Calls on one
+ * context are not re-entrant; distinct contexts are independent.
+ */
+#ifndef MCL_H
+#define MCL_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCL_ABI_VERSION 1
+
+enum mcl_error {
+    MCL_OK = 0,
+    MCL_E_ARG = 1,        /* invalid argument or inconsistent mesh */
+    MCL_E_DEVICE = 2,     /* HIP runtime error */
+    MCL_E_STATE = 3,      /* call out of order (e.g. solve before mesh upload) */
+    MCL_E_NUMERIC = 4,    /* NaN/Inf detected in a solve */
+    MCL_E_UNSUPPORTED = 5 /* option not available */
+};
+
+/* Inner solvers (the reference's `solver` key, Solver.c:82-89). */
+enum mcl_solver { MCL_SOLVER_JC = 0, MCL_SOLVER_GS = 1, MCL_SOLVER_CG = 2, MCL_SOLVER_MG = 3 };
+
+/* Schur-complement diagonal use in the Uzawa iteration. */
+enum mcl_schur {
+    MCL_SCHUR_REFERENCE = 0, /* z = BAB * r, as Solver.c:133-134 */
+    MCL_SCHUR_DIVIDE = 1     /* z = r / BAB */
+};
+
+/* Arithmetic of the Stokes solve. */
+enum mcl_precision {
+    MCL_PREC_FP64 = 0,       /* FP64 throughout, reference operation order */
+    MCL_PREC_MIXED = 1       /* FP32 inner solves inside FP64 iterative refinement */
+};
+
+/* Fields that can be moved between host and device (per quadtree level). */
+enum mcl_field {
+    MCL_FIELD_T = 0,          /* nno   temperature            G.T[lev] */
+    MCL_FIELD_VIS = 1,        /* nno   viscosity              G.vis[lev] */
+    MCL_FIELD_BUOYANCY = 2,   /* nno   buoyancy               G.buoyancy[lev] */
+    MCL_FIELD_V = 3,          /* 2*nno nodal velocity [d][node] G.V[lev] */
+    MCL_FIELD_D = 4,          /* neq   velocity solution      G.d[lev] */
+    MCL_FIELD_P = 5,          /* nel   pressure               G.p[lev] */
+    MCL_FIELD_F = 6,          /* neq   force vector           G.F[lev] */
+    MCL_FIELD_KDIAG = 7,      /* neq   diagonal of K          G.Kdiag[lev] */
+    MCL_FIELD_STRESS = 8,     /* nno   nodal stress           G.stress[lev] */
+    MCL_FIELD_HEAT_D = 9,     /* neqT  heat solution          G.heat[lev].d */
+    MCL_FIELD_HEAT_DDOT = 10, /* neqT  heat rate              G.heat[lev].ddot */
+    MCL_FIELD_HEAT_F = 11,    /* neqT  heat force             G.heat[lev].F */
+    MCL_FIELD_TAU = 12        /* 2*nel SUPG tau [d][e]        G.heat[lev].tau */
+};
+
+/* Assembled matrices that can be read back (row-slot form, Mesh.c:455-511). */
+enum mcl_matrix {
+    MCL_MATRIX_K = 0,         /* velocity stiffness, neq rows   G.Krow_* */
+    MCL_MATRIX_G = 1,         /* gradient, neq rows, element columns  G.Grow_* */
+    MCL_MATRIX_HEAT_K = 2,    /* heat advection-diffusion, neqT rows */
+    MCL_MATRIX_HEAT_C = 3     /* heat (SUPG) capacity, neqT rows */
+};
+
+/* One quadtree level of the reference mesh, in the reference's layouts
+ * (global_variables.h:315-422), flattened row-major:
+ *   Mcode[e]                 (morton << 4) | element level      G.Mcode[lev]
+ *   ien[a*nel+e]             global node numbers, a = 0..3       G.ien[lev][a]
+ *   node[i]                  sorted global node ids              G.node[lev]
+ *   id[d*nno+i]              velocity equation, -1 = fixed       G.id[lev][d]
+ *   lm[(d*4+a)*nel+e]        id at element corner                G.lm[lev][d][a]
+ *   heat_id[i], heat_lm[a*nel+e]                                 G.heat[lev].id/lm
+ *   v_bc[d*nno+i], v_bc_val  'V' fixed, 'S' free slip, 'N'       G.bc.v_bc[lev][d]
+ *   T_bc[i], T_bc_val        'T' fixed, 'F' flux, 'N'            G.bc.T_bc[lev]
+ *   X[d*nno+i]               node coordinates                    G.X[lev][d]
+ *   dx[l], dz[l]             element sizes of quadtree level l   G.dx, G.dz
+ *   h_ni[k], h_ci[c*nh+k]    hanging node k and its two parents  G.hangle[lev].ni/ci
+ * MG transfer maps (only for MCL_SOLVER_MG, else NULL), from Connection.c:708-1500:
+ *   interp_n[u], interp_eq[k*neq+u]  eq u of this level <- mean of interp_n[u]
+ *                            eqs of the level below (-1 = fixed)  G.interp[lev]
+ *   proj_l[k], proj_u[k]     eq proj_l[k] of the level below <- eq proj_u[k] of
+ *                            this level (injection)               G.project[lev-1]
+ */
+typedef struct mcl_mesh_view {
+    int L;            /* G.max_level */
+    int lev;          /* level index of this view */
+    int nel, nno, neq, neqT;
+    int gnox, gnoz;
+    const int *Mcode;
+    const int *ien;
+    const int *node;
+    const int *id;
+    const int *lm;
+    const int *heat_id;
+    const int *heat_lm;
+    const signed char *v_bc;
+    const double *v_bc_val;
+    const signed char *T_bc;
+    const double *T_bc_val;
+    const double *X;
+    const double *dx;
+    const double *dz;
+    int nh;
+    const int *h_ni;
+    const int *h_ci;
+    const int *interp_n;
+    const int *interp_eq;
+    int nproj;
+    const int *proj_l;
+    const int *proj_u;
+} mcl_mesh_view;
+
+/* The reference's run-time controls used on the hot path (Input.c:3-98), plus
+ * the options this library adds (schur, precision, warm_start). */
+typedef struct mcl_control {
+    double Ra;                 /* Rayleigh */
+    double A, refT, lm_jump;   /* viscosity law, Viscosity.c:13-16 */
+    double accuracy;           /* inner-solver tolerance (and SUPG small-alpha threshold) */
+    double citcom_accu;        /* Uzawa dv/dp tolerance, citcom_solver_accu */
+    double theta;              /* heat time weight */
+    double Q;                  /* internal heating */
+    int max_citcom_cycle;
+    int max_cg_cycle;
+    int cg_precondition;
+    int check_v_only;
+    int solver;                /* enum mcl_solver */
+    int auto_accuracy;
+    int zero_P;
+    int AMR;
+    int mg_cycle, mg_finest_iteration, mg_coarse_iteration;
+    int schur;                 /* enum mcl_schur */
+    int precision;             /* enum mcl_precision */
+    int warm_start;            /* 1: keep d and p between steps (overrides zero_P) */
+    double inner_rtol;         /* MCL_PREC_MIXED: relative tolerance of each inner solve */
+} mcl_control;
+
+typedef struct mcl_stokes_stats {
+    int uzawa_cycles;
+    int converged;             /* dv, dp below citcom_accu before the cycle cap */
+    double dv, dp;
+    long long inner_iterations;
+    long long spmv;
+    double accuracy_used;
+    double res_momentum;       /* |F - K d - G p| / |F| after the solve */
+    double res_continuity;     /* |G^T d| / |F| after the solve */
+    double device_ms;          /* device time of the solve */
+} mcl_stokes_stats;
+
+typedef struct mcl_heat_stats {
+    int cg_cycles;
+    int init_cg_cycles;
+    double dt;
+    double model_time;
+    double device_ms;
+} mcl_heat_stats;
+
+typedef struct mcl_ctx mcl_ctx;
+
+/* Library / context lifetime. */
+int mcl_abi_version(void);
+int mcl_create(int device, mcl_ctx **out);
+void mcl_destroy(mcl_ctx *ctx);
+const char *mcl_last_error(const mcl_ctx *ctx);
+
+/* Mesh (Main.c:33-52 setup products).  levels[0..nlev-1] = G.min_level..G.max_level-1. */
+int mcl_mesh_upload(mcl_ctx *ctx, const mcl_mesh_view *levels, int nlev);
+int mcl_set_control(mcl_ctx *ctx, const mcl_control *control);
+
+/* Host <-> device transfer of fields (count elements of double). */
+int mcl_set_field(mcl_ctx *ctx, int lev, int field, const double *src, size_t count);
+int mcl_get_field(mcl_ctx *ctx, int lev, int field, double *dst, size_t count);
+/* Assembled matrix in row-slot form: ncol[row], col/val[row*width+slot]. */
+int mcl_get_matrix(mcl_ctx *ctx, int lev, int matrix, int width, int *ncol, int *col, double *val);
+int mcl_set_model_time(mcl_ctx *ctx, double model_time);
+int mcl_get_model_time(mcl_ctx *ctx, double *model_time);
+
+/* The per-step hot path, in the reference's call order (Main.c:58-64). */
+int mcl_buoyancy_force(mcl_ctx *ctx);                 /* Buoyancy.c:3-22 */
+int mcl_viscosity(mcl_ctx *ctx);                      /* Viscosity.c:3-24 */
+int mcl_velocity_pressure_KGF_matrix(mcl_ctx *ctx);   /* Build_matrix.c:4-40 */
+int mcl_solve_velocity_pressure(mcl_ctx *ctx, mcl_stokes_stats *stats); /* Solver.c:3-12 */
+int mcl_update_temperature(mcl_ctx *ctx, mcl_heat_stats *stats);        /* Update_temperature.c:3-133 */
+
+/* Synchronise the context's stream. */
+int mcl_synchronize(mcl_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

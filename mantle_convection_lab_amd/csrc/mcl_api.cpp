@@ -1,0 +1,986 @@
+// Host side of libmcl_hip.so: the C-ABI declared in include/mcl.h.
+//
+// mcl_mesh_upload turns the reference's mesh arrays (one view per quadtree
+// level) into the device layout of LevelDev: local connectivity, node ->
+// element adjacency, sorted row patterns with per-(element, dof) slot maps for
+// atomic-free row assembly, element -> row lists for G^T, the buoyancy row
+// contribution lists, and the static time-step terms.  All integer work here
+// runs once per mesh.  The per-step entry points only launch kernels on the
+// context's stream; nothing is copied between host and device unless the
+// caller asks for it (mcl_set_field / mcl_get_field).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mcl.h"
+#include "mcl_kernels.h"
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Level {
+    // host copies of the views needed later
+    int L = 0, lev = 0, nel = 0, nno = 0, neq = 0, neqT = 0, gnox = 0, gnoz = 0;
+    std::vector<int> Mcode, ienl, node, id, heat_id;
+    std::vector<signed char> v_bc, T_bc;
+    std::vector<double> v_bc_val, T_bc_val, X;
+    std::vector<int> Kncol_h, Kcol_h, Tncol_h, Tcol_h;   // patterns (slot-major)
+    int WK = 0, WT = 0;
+    double Llast0 = 0, Llast1 = 0, diff_dt = 0;
+    LevelDev dev{};
+    // device fields
+    double *T = nullptr, *vis = nullptr, *buoy = nullptr, *V = nullptr, *d = nullptr, *p = nullptr;
+    double *stress = nullptr, *es = nullptr, *area = nullptr, *avg = nullptr;
+    // buoyancy row lists
+    int *brow_ptr = nullptr, *brow_e = nullptr;
+    signed char *brow_xi = nullptr, *brow_eta = nullptr;
+    std::vector<void *> owned;
+};
+
+}  // namespace
+
+struct mcl_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int nlev = 0;
+    std::vector<Level> levels;
+    mcl_control ctl{};
+    bool have_ctl = false;
+    bool first_heat = true;
+    bool been_here_dp = false;
+    int max_coop_blocks = 0;
+    int num_cu = 0;
+    // solver work space (finest level)
+    GridSync gs{};
+    RedBuf rb{};
+    DevStats *stats = nullptr;       // device
+    double *work = nullptr;          // device scratch vectors
+    size_t work_doubles = 0;
+    double *model_time = nullptr;    // device scalar
+    double *resid = nullptr;         // device [3]
+    // heat work (finest level)
+    double *tau = nullptr, *hK = nullptr, *hC = nullptr, *hA = nullptr, *hF = nullptr;
+    double *hd = nullptr, *hdd = nullptr, *hd0 = nullptr, *hB = nullptr;
+    std::vector<void *> owned;
+};
+
+static int fail(mcl_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                                   \
+    do {                                                                                                    \
+        hipError_t e_ = (call);                                                                             \
+        if (e_ != hipSuccess) return fail(ctx, MCL_E_DEVICE, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                          __FILE__, __LINE__);                                              \
+    } while (0)
+
+template <typename T>
+static int dev_alloc(mcl_ctx *c, std::vector<void *> &owner, T **out, size_t count)
+{
+    void *p = nullptr;
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail(c, MCL_E_DEVICE, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    hipMemset(p, 0, bytes);
+    owner.push_back(p);
+    *out = (T *)p;
+    return 0;
+}
+
+template <typename T>
+static int dev_upload(mcl_ctx *c, std::vector<void *> &owner, const T **out, const std::vector<T> &h)
+{
+    T *p = nullptr;
+    int rc = dev_alloc(c, owner, &p, h.size());
+    if (rc) return rc;
+    if (!h.empty()) {
+        hipError_t e = hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail(c, MCL_E_DEVICE, "hipMemcpy: %s", hipGetErrorString(e));
+    }
+    *out = p;
+    return 0;
+}
+
+static void free_all(std::vector<void *> &v)
+{
+    for (void *p : v) hipFree(p);
+    v.clear();
+}
+
+// Shape tables computed with the expressions of Shape_function.c:17-121.
+static void shape_tables(ShapeTables *s)
+{
+    const double pts[4][2] = {{-1.0, -1.0}, {-1.0, 1.0}, {1.0, 1.0}, {1.0, -1.0}};
+    volatile double beta = 1.0 / std::sqrt(3.0);
+    for (int g = 0; g < 4; g++) {
+        volatile double xi = pts[g][0] * beta, eta = pts[g][1] * beta;
+        s->N[0][g] = 0.25 * (1 - xi) * (1 - eta);
+        s->N[1][g] = 0.25 * (1 - xi) * (1 + eta);
+        s->N[2][g] = 0.25 * (1 + xi) * (1 + eta);
+        s->N[3][g] = 0.25 * (1 + xi) * (1 - eta);
+        s->dxi[0][g] = 0.25 * (-1.0) * (1 - eta);
+        s->dxi[1][g] = 0.25 * (-1.0) * (1 + eta);
+        s->dxi[2][g] = 0.25 * (1.0) * (1 + eta);
+        s->dxi[3][g] = 0.25 * (1.0) * (1 - eta);
+        s->deta[0][g] = 0.25 * (1.0 - xi) * (-1.0);
+        s->deta[1][g] = 0.25 * (1.0 - xi) * (1.0);
+        s->deta[2][g] = 0.25 * (1.0 + xi) * (1.0);
+        s->deta[3][g] = 0.25 * (1.0 + xi) * (-1.0);
+    }
+    s->dxic[0] = -0.25; s->dxic[1] = -0.25; s->dxic[2] = 0.25; s->dxic[3] = 0.25;
+    s->detac[0] = -0.25; s->detac[1] = 0.25; s->detac[2] = 0.25; s->detac[3] = -0.25;
+}
+
+extern "C" int mcl_abi_version(void) { return MCL_ABI_VERSION; }
+
+extern "C" const char *mcl_last_error(const mcl_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int mcl_create(int device, mcl_ctx **out)
+{
+    if (!out) return MCL_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return MCL_E_DEVICE;
+    mcl_ctx *c = new mcl_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) { delete c; return MCL_E_DEVICE; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return MCL_E_DEVICE; }
+    c->num_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return MCL_E_DEVICE; }
+    ShapeTables st;
+    shape_tables(&st);
+    if (mcl_upload_shape(&st) != hipSuccess) { delete c; return MCL_E_DEVICE; }
+    int per_cu_u = 0, per_cu_h = 0;
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_u, (const void *)k_uzawa_fp64, MCL_BLOCK, 0);
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_h, (const void *)k_heat_fp64, MCL_BLOCK, 0);
+    c->max_coop_blocks = std::min(std::max(1, std::min(per_cu_u, per_cu_h)) * c->num_cu, MCL_MAX_WG);
+    // grid barrier and reduction buffers
+    unsigned *sync = nullptr;
+    if (dev_alloc(c, c->owned, &sync, 4)) { delete c; return MCL_E_DEVICE; }
+    c->gs.count = sync;
+    c->gs.gen = sync + 1;
+    c->gs.abort = sync + 2;
+    if (dev_alloc(c, c->owned, &c->rb.part, (size_t)MCL_RED_SLOTS * MCL_MAX_WG) ||
+        dev_alloc(c, c->owned, &c->rb.pmax, (size_t)MCL_RED_SLOTS * MCL_MAX_WG) ||
+        dev_alloc(c, c->owned, &c->stats, 1) || dev_alloc(c, c->owned, &c->model_time, 1) ||
+        dev_alloc(c, c->owned, &c->resid, 4)) {
+        free_all(c->owned);
+        delete c;
+        return MCL_E_DEVICE;
+    }
+    *out = c;
+    return 0;
+}
+
+extern "C" void mcl_destroy(mcl_ctx *c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto &L : c->levels) free_all(L.owned);
+    free_all(c->owned);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+// Local node index of a global node id (node[] is sorted ascending).
+static int local_of(const std::vector<int> &node, int g)
+{
+    auto it = std::lower_bound(node.begin(), node.end(), g);
+    if (it == node.end() || *it != g) return -1;
+    return (int)(it - node.begin());
+}
+
+static int build_level(mcl_ctx *c, const mcl_mesh_view &v, Level &Lv)
+{
+    const int nel = v.nel, nno = v.nno, neq = v.neq, neqT = v.neqT;
+    if (nel <= 0 || nno <= 0 || neq < 0 || neqT < 0 || v.L <= 0 || v.L > MCL_MAX_LEVELS)
+        return fail(c, MCL_E_ARG, "level %d: bad sizes", v.lev);
+    if (!v.Mcode || !v.ien || !v.node || !v.id || !v.lm || !v.heat_id || !v.heat_lm || !v.v_bc || !v.v_bc_val ||
+        !v.T_bc || !v.T_bc_val || !v.X || !v.dx || !v.dz)
+        return fail(c, MCL_E_ARG, "level %d: missing array", v.lev);
+    Lv.L = v.L; Lv.lev = v.lev; Lv.nel = nel; Lv.nno = nno; Lv.neq = neq; Lv.neqT = neqT;
+    Lv.gnox = v.gnox; Lv.gnoz = v.gnoz;
+    Lv.Mcode.assign(v.Mcode, v.Mcode + nel);
+    Lv.node.assign(v.node, v.node + nno);
+    for (int i = 1; i < nno; i++)
+        if (Lv.node[i] <= Lv.node[i - 1]) return fail(c, MCL_E_ARG, "level %d: node list not sorted", v.lev);
+    Lv.id.assign(v.id, v.id + 2 * (size_t)nno);
+    Lv.heat_id.assign(v.heat_id, v.heat_id + nno);
+    Lv.v_bc.assign(v.v_bc, v.v_bc + 2 * (size_t)nno);
+    Lv.v_bc_val.assign(v.v_bc_val, v.v_bc_val + 2 * (size_t)nno);
+    Lv.T_bc.assign(v.T_bc, v.T_bc + nno);
+    Lv.T_bc_val.assign(v.T_bc_val, v.T_bc_val + nno);
+    Lv.X.assign(v.X, v.X + 2 * (size_t)nno);
+
+    std::vector<int> elev(nel), ienl(4 * (size_t)nel);
+    for (int e = 0; e < nel; e++) {
+        elev[e] = v.Mcode[e] & 15;
+        if (elev[e] >= v.L) return fail(c, MCL_E_ARG, "element %d: level %d >= L", e, elev[e]);
+        for (int a = 0; a < 4; a++) {
+            int li = local_of(Lv.node, v.ien[a * nel + e]);
+            if (li < 0) return fail(c, MCL_E_ARG, "element %d corner %d: node not in node list", e, a);
+            ienl[a * (size_t)nel + e] = li;
+        }
+    }
+    Lv.ienl = ienl;
+    // node -> elements (ascending), corner index
+    std::vector<int> cnt(nno, 0), p2n_el(4 * (size_t)nno, -1), p2n_a(4 * (size_t)nno, -1);
+    for (int e = 0; e < nel; e++)
+        for (int a = 0; a < 4; a++) {
+            int n = ienl[a * (size_t)nel + e];
+            if (cnt[n] >= 4) return fail(c, MCL_E_ARG, "node %d in more than 4 elements", n);
+            p2n_el[cnt[n] * (size_t)nno + n] = e;
+            p2n_a[cnt[n] * (size_t)nno + n] = a;
+            cnt[n]++;
+        }
+    // equations -> node, dof (ids numbered in node order)
+    std::vector<int> veq_node(std::max(neq, 1), -1), veq_dof(std::max(neq, 1), -1), heq_node(std::max(neqT, 1), -1);
+    for (int i = 0; i < nno; i++) {
+        for (int d = 0; d < 2; d++) {
+            int q = v.id[d * nno + i];
+            if (q >= neq) return fail(c, MCL_E_ARG, "id out of range");
+            if (q >= 0) { veq_node[q] = i; veq_dof[q] = d; }
+        }
+        int t = v.heat_id[i];
+        if (t >= neqT) return fail(c, MCL_E_ARG, "heat id out of range");
+        if (t >= 0) heq_node[t] = i;
+    }
+    for (int q = 0; q < neq; q++)
+        if (veq_node[q] < 0) return fail(c, MCL_E_ARG, "velocity equation %d has no node", q);
+    for (int q = 0; q < neqT; q++)
+        if (heq_node[q] < 0) return fail(c, MCL_E_ARG, "heat equation %d has no node", q);
+    // lm consistency with id
+    for (int e = 0; e < nel; e++)
+        for (int d = 0; d < 2; d++)
+            for (int a = 0; a < 4; a++)
+                if (v.lm[(d * 4 + a) * (size_t)nel + e] != v.id[d * nno + ienl[a * (size_t)nel + e]])
+                    return fail(c, MCL_E_ARG, "lm inconsistent with id at element %d", e);
+    // velocity row patterns: sorted unique columns over the node's elements
+    std::vector<std::vector<int>> rows(neq);
+    int WK = 0;
+    for (int P = 0; P < neq; P++) {
+        int n = veq_node[P];
+        auto &r = rows[P];
+        for (int k = 0; k < cnt[n]; k++) {
+            int e = p2n_el[k * (size_t)nno + n];
+            for (int q = 0; q < 8; q++) {
+                int Q = v.lm[((q & 1) * 4 + (q >> 1)) * (size_t)nel + e];
+                if (Q >= 0) r.push_back(Q);
+            }
+        }
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end()), r.end());
+        WK = std::max<int>(WK, (int)r.size());
+    }
+    if (WK > 127) return fail(c, MCL_E_ARG, "velocity row too long (%d)", WK);
+    std::vector<int> Kcol((size_t)std::max(WK, 1) * std::max(neq, 1), 0), Kncol(std::max(neq, 1), 0);
+    std::vector<signed char> Kslot(32 * (size_t)std::max(neq, 1), -1);
+    for (int P = 0; P < neq; P++) {
+        auto &r = rows[P];
+        Kncol[P] = (int)r.size();
+        for (size_t s = 0; s < r.size(); s++) Kcol[s * neq + P] = r[s];
+        for (size_t s = r.size(); s < (size_t)WK; s++) Kcol[s * neq + P] = P;  // padding reads x[P] * 0
+        int n = veq_node[P];
+        for (int k = 0; k < cnt[n]; k++) {
+            int e = p2n_el[k * (size_t)nno + n];
+            for (int q = 0; q < 8; q++) {
+                int Q = v.lm[((q & 1) * 4 + (q >> 1)) * (size_t)nel + e];
+                if (Q < 0) continue;
+                int s = (int)(std::lower_bound(r.begin(), r.end(), Q) - r.begin());
+                Kslot[(k * 8 + q) * (size_t)neq + P] = (signed char)s;
+            }
+        }
+    }
+    Lv.WK = WK;
+    Lv.Kncol_h = Kncol;
+    Lv.Kcol_h = Kcol;
+    // element -> free rows ascending (G^T)
+    std::vector<int> gt_row(8 * (size_t)nel, -1);
+    std::vector<signed char> gt_p(8 * (size_t)nel, 0);
+    for (int e = 0; e < nel; e++) {
+        std::vector<std::pair<int, int>> rp;
+        for (int d = 0; d < 2; d++)
+            for (int a = 0; a < 4; a++) {
+                int P = v.lm[(d * 4 + a) * (size_t)nel + e];
+                if (P >= 0) rp.push_back({P, 2 * a + d});
+            }
+        std::sort(rp.begin(), rp.end());
+        for (size_t t = 0; t < rp.size(); t++) {
+            gt_row[t * nel + e] = rp[t].first;
+            gt_p[t * nel + e] = (signed char)rp[t].second;
+        }
+    }
+    // heat row patterns
+    std::vector<std::vector<int>> hrows(neqT);
+    int WT = 0;
+    for (int P = 0; P < neqT; P++) {
+        int n = heq_node[P];
+        auto &r = hrows[P];
+        for (int k = 0; k < cnt[n]; k++) {
+            int e = p2n_el[k * (size_t)nno + n];
+            for (int b = 0; b < 4; b++) {
+                int Q = v.heat_lm[b * (size_t)nel + e];
+                if (Q >= 0) r.push_back(Q);
+            }
+        }
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end()), r.end());
+        WT = std::max<int>(WT, (int)r.size());
+    }
+    std::vector<int> Tcol((size_t)std::max(WT, 1) * std::max(neqT, 1), 0), Tncol(std::max(neqT, 1), 0),
+        Tdiag(std::max(neqT, 1), 0);
+    std::vector<signed char> Tslot(16 * (size_t)std::max(neqT, 1), -1);
+    for (int P = 0; P < neqT; P++) {
+        auto &r = hrows[P];
+        Tncol[P] = (int)r.size();
+        for (size_t s = 0; s < r.size(); s++) Tcol[s * neqT + P] = r[s];
+        for (size_t s = r.size(); s < (size_t)WT; s++) Tcol[s * neqT + P] = P;
+        auto it = std::lower_bound(r.begin(), r.end(), P);
+        if (it == r.end() || *it != P) return fail(c, MCL_E_ARG, "heat row %d has no diagonal", P);
+        Tdiag[P] = (int)(it - r.begin());
+        int n = heq_node[P];
+        for (int k = 0; k < cnt[n]; k++) {
+            int e = p2n_el[k * (size_t)nno + n];
+            for (int b = 0; b < 4; b++) {
+                int Q = v.heat_lm[b * (size_t)nel + e];
+                if (Q < 0) continue;
+                Tslot[(k * 4 + b) * (size_t)neqT + P] =
+                    (signed char)(std::lower_bound(r.begin(), r.end(), Q) - r.begin());
+            }
+        }
+    }
+    Lv.WT = WT;
+    Lv.Tncol_h = Tncol;
+    Lv.Tcol_h = Tcol;
+    // buoyancy row-average contribution lists, in the reference's order
+    // (get_node_horiz_avg, Parallel.c:354-437)
+    std::vector<std::vector<int>> be(v.gnoz);
+    std::vector<std::vector<signed char>> bxi(v.gnoz), beta(v.gnoz);
+    for (int e = 0; e < nel; e++) {
+        int n0 = v.ien[0 * nel + e], n2 = v.ien[2 * nel + e];
+        int ix0 = n0 / v.gnoz, iz0 = n0 - ix0 * v.gnoz, ix1 = n2 / v.gnoz, iz1 = n2 - ix1 * v.gnoz;
+        for (int i = ix0; i < ix1; i++) {
+            int xi = (2 * i - ix0 - ix1) / (ix1 - ix0);
+            for (int k = iz0; k < iz1; k++) {
+                int eta = (2 * k - iz0 - iz1) / (iz1 - iz0);
+                be[k].push_back(e); bxi[k].push_back((signed char)xi); beta[k].push_back((signed char)eta);
+            }
+        }
+        if (iz1 == v.gnoz - 1)
+            for (int i = ix0; i < ix1; i++) {
+                int xi = (2 * i - ix0 - ix1) / (ix1 - ix0);
+                be[iz1].push_back(e); bxi[iz1].push_back((signed char)xi); beta[iz1].push_back(1);
+            }
+        if (ix1 == v.gnox - 1)
+            for (int k = iz0; k < iz1; k++) {
+                int eta = (2 * k - iz0 - iz1) / (iz1 - iz0);
+                be[k].push_back(e); bxi[k].push_back(1); beta[k].push_back((signed char)eta);
+            }
+        if (n2 == v.gnox * v.gnoz - 1) {
+            be[v.gnoz - 1].push_back(-1); bxi[v.gnoz - 1].push_back(0); beta[v.gnoz - 1].push_back(0);
+        }
+    }
+    std::vector<int> bptr(v.gnoz + 1, 0), bent;
+    std::vector<signed char> bx, bz;
+    for (int k = 0; k < v.gnoz; k++) {
+        bent.insert(bent.end(), be[k].begin(), be[k].end());
+        bx.insert(bx.end(), bxi[k].begin(), bxi[k].end());
+        bz.insert(bz.end(), beta[k].begin(), beta[k].end());
+        bptr[k + 1] = (int)bent.size();
+    }
+    // static dt terms of advection_dt (Update_temperature.c:135-160)
+    double diff = 1e10, L0 = 0.0, L1 = 0.0;
+    for (int e = 0; e < nel; e++) {
+        L0 = v.dx[elev[e]];
+        L1 = v.dz[elev[e]];
+        double t0 = L0 * L0, t1 = L1 * L1;
+        if (diff > t0) diff = t0;
+        if (diff > t1) diff = t1;
+    }
+    Lv.Llast0 = L0; Lv.Llast1 = L1; Lv.diff_dt = diff;
+    // level tables: sizes and element gradient vectors (get_element_G)
+    ShapeTables sh;
+    shape_tables(&sh);
+    std::vector<double> ldx(MCL_MAX_LEVELS, 0.0), ldz(MCL_MAX_LEVELS, 0.0), gtab(MCL_MAX_LEVELS * 8, 0.0);
+    for (int l = 0; l < v.L; l++) {
+        ldx[l] = v.dx[l];
+        ldz[l] = v.dz[l];
+        volatile double dxdxi = 0.5 * v.dx[l], dzdeta = 0.5 * v.dz[l];
+        const double zero = 0.0;
+        for (int a = 0; a < 4; a++) {
+            volatile double g0 = 0.0, g1 = 0.0;
+            for (int gs = 0; gs < 4; gs++) {
+                volatile double t = sh.dxi[a][gs] * dzdeta;
+                volatile double u = sh.deta[a][gs] * zero;
+                g0 = g0 - (t - u);
+            }
+            for (int gs = 0; gs < 4; gs++) {
+                volatile double t = sh.deta[a][gs] * dxdxi;
+                volatile double u = sh.dxi[a][gs] * zero;
+                g1 = g1 - (t - u);
+            }
+            gtab[l * 8 + 2 * a] = g0;
+            gtab[l * 8 + 2 * a + 1] = g1;
+        }
+    }
+    // hanging nodes: parallel when no parent is itself a hanging node
+    int nh = v.nh > 0 ? v.nh : 0, h_serial = 0;
+    std::vector<int> h_ni(std::max(nh, 1), 0), h_ci(2 * (size_t)std::max(nh, 1), 0);
+    if (nh > 0) {
+        if (!v.h_ni || !v.h_ci) return fail(c, MCL_E_ARG, "hanging arrays missing");
+        std::vector<char> is_h(nno, 0);
+        for (int k = 0; k < nh; k++) {
+            h_ni[k] = v.h_ni[k];
+            h_ci[k] = v.h_ci[k];
+            h_ci[nh + k] = v.h_ci[nh + k];
+            if (h_ni[k] >= 0) is_h[h_ni[k]] = 1;
+        }
+        for (int k = 0; k < nh; k++)
+            if (is_h[h_ci[k]] || is_h[h_ci[nh + k]]) h_serial = 1;
+    }
+
+    LevelDev &D = Lv.dev;
+    std::vector<void *> &o = Lv.owned;
+    D.L = v.L; D.lev = v.lev; D.nel = nel; D.nno = nno; D.neq = neq; D.neqT = neqT; D.gnox = v.gnox; D.gnoz = v.gnoz;
+    std::vector<int> lm(v.lm, v.lm + 8 * (size_t)nel);
+    std::vector<int> nodev(Lv.node), idv(Lv.id), hid(Lv.heat_id);
+    int rc = 0;
+    rc |= dev_upload(c, o, &D.lvl_dx, ldx);
+    rc |= dev_upload(c, o, &D.lvl_dz, ldz);
+    rc |= dev_upload(c, o, &D.gtab, gtab);
+    rc |= dev_upload(c, o, &D.ienl, ienl);
+    rc |= dev_upload(c, o, &D.elev, elev);
+    rc |= dev_upload(c, o, &D.lm, lm);
+    rc |= dev_upload(c, o, &D.gt_row, gt_row);
+    rc |= dev_upload(c, o, &D.gt_p, gt_p);
+    rc |= dev_upload(c, o, &D.node, nodev);
+    rc |= dev_upload(c, o, &D.p2n_cnt, cnt);
+    rc |= dev_upload(c, o, &D.p2n_el, p2n_el);
+    rc |= dev_upload(c, o, &D.p2n_a, p2n_a);
+    rc |= dev_upload(c, o, &D.id, idv);
+    rc |= dev_upload(c, o, &D.heat_id, hid);
+    rc |= dev_upload(c, o, &D.v_bc, Lv.v_bc);
+    rc |= dev_upload(c, o, &D.v_bc_val, Lv.v_bc_val);
+    rc |= dev_upload(c, o, &D.T_bc, Lv.T_bc);
+    rc |= dev_upload(c, o, &D.T_bc_val, Lv.T_bc_val);
+    rc |= dev_upload(c, o, &D.X, Lv.X);
+    rc |= dev_upload(c, o, &D.veq_node, veq_node);
+    rc |= dev_upload(c, o, &D.veq_dof, veq_dof);
+    D.WK = WK;
+    rc |= dev_upload(c, o, &D.Kcol, Kcol);
+    rc |= dev_upload(c, o, &D.Kncol, Kncol);
+    rc |= dev_upload(c, o, &D.Kslot, Kslot);
+    rc |= dev_alloc(c, o, &D.Kval, (size_t)std::max(WK, 1) * std::max(neq, 1));
+    rc |= dev_alloc(c, o, &D.Kdiag, std::max(neq, 1));
+    rc |= dev_alloc(c, o, &D.F, std::max(neq, 1));
+    rc |= dev_upload(c, o, &D.heq_node, heq_node);
+    D.WT = WT;
+    rc |= dev_upload(c, o, &D.Tcol, Tcol);
+    rc |= dev_upload(c, o, &D.Tncol, Tncol);
+    rc |= dev_upload(c, o, &D.Tslot, Tslot);
+    rc |= dev_upload(c, o, &D.Tdiag_slot, Tdiag);
+    D.nh = nh;
+    D.h_serial = h_serial;
+    rc |= dev_upload(c, o, &D.h_ni, h_ni);
+    rc |= dev_upload(c, o, &D.h_ci, h_ci);
+    const int *cp;
+    const signed char *sp;
+    rc |= dev_upload(c, o, &cp, bptr); Lv.brow_ptr = (int *)cp;
+    rc |= dev_upload(c, o, &cp, bent); Lv.brow_e = (int *)cp;
+    rc |= dev_upload(c, o, &sp, bx); Lv.brow_xi = (signed char *)sp;
+    rc |= dev_upload(c, o, &sp, bz); Lv.brow_eta = (signed char *)sp;
+    rc |= dev_alloc(c, o, &Lv.T, nno);
+    rc |= dev_alloc(c, o, &Lv.vis, nno);
+    rc |= dev_alloc(c, o, &Lv.buoy, nno);
+    rc |= dev_alloc(c, o, &Lv.V, 2 * (size_t)nno);
+    rc |= dev_alloc(c, o, &Lv.d, std::max(neq, 1));
+    rc |= dev_alloc(c, o, &Lv.p, nel);
+    rc |= dev_alloc(c, o, &Lv.stress, nno);
+    rc |= dev_alloc(c, o, &Lv.es, nel);
+    rc |= dev_alloc(c, o, &Lv.area, nel);
+    rc |= dev_alloc(c, o, &Lv.avg, v.gnoz);
+    return rc ? (c->err.empty() ? fail(c, MCL_E_DEVICE, "device allocation failed") : MCL_E_DEVICE) : 0;
+}
+
+extern "C" int mcl_mesh_upload(mcl_ctx *c, const mcl_mesh_view *levels, int nlev)
+{
+    if (!c) return MCL_E_ARG;
+    if (!levels || nlev <= 0 || nlev > MCL_MAX_LEVELS) return fail(c, MCL_E_ARG, "bad level list");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (auto &L : c->levels) free_all(L.owned);
+    c->levels.clear();
+    c->levels.resize(nlev);
+    for (int l = 0; l < nlev; l++) {
+        int rc = build_level(c, levels[l], c->levels[l]);
+        if (rc) {
+            for (auto &L : c->levels) free_all(L.owned);
+            c->levels.clear();
+            c->nlev = 0;
+            return rc;
+        }
+    }
+    c->nlev = nlev;
+    // finest-level solver work space
+    Level &F = c->levels[nlev - 1];
+    for (void *p : {(void *)c->work, (void *)c->tau, (void *)c->hK, (void *)c->hC, (void *)c->hA, (void *)c->hF,
+                    (void *)c->hd, (void *)c->hdd, (void *)c->hd0, (void *)c->hB}) {
+        if (p) {
+            hipFree(p);
+            c->owned.erase(std::remove(c->owned.begin(), c->owned.end(), p), c->owned.end());
+        }
+    }
+    size_t nv = std::max(F.neq, 1), np = F.nel, nt = std::max(F.neqT, 1);
+    c->work_doubles = 6 * nv + 9 * np + 4 * std::max(nv, nt);
+    int rc = dev_alloc(c, c->owned, &c->work, c->work_doubles);
+    size_t wt = (size_t)std::max(F.WT, 1) * nt;
+    rc |= dev_alloc(c, c->owned, &c->tau, 2 * np);
+    rc |= dev_alloc(c, c->owned, &c->hK, wt);
+    rc |= dev_alloc(c, c->owned, &c->hC, wt);
+    rc |= dev_alloc(c, c->owned, &c->hA, wt);
+    rc |= dev_alloc(c, c->owned, &c->hF, nt);
+    rc |= dev_alloc(c, c->owned, &c->hd, nt);
+    rc |= dev_alloc(c, c->owned, &c->hdd, nt);
+    rc |= dev_alloc(c, c->owned, &c->hd0, nt);
+    rc |= dev_alloc(c, c->owned, &c->hB, nt);
+    if (rc) return MCL_E_DEVICE;
+    c->first_heat = true;
+    c->been_here_dp = false;
+    return 0;
+}
+
+extern "C" int mcl_set_control(mcl_ctx *c, const mcl_control *ctl)
+{
+    if (!c || !ctl) return MCL_E_ARG;
+    if (ctl->solver < 0 || ctl->solver > 3) return fail(c, MCL_E_ARG, "solver %d", ctl->solver);
+    if (ctl->solver == MCL_SOLVER_JC || ctl->solver == MCL_SOLVER_GS || ctl->solver == MCL_SOLVER_MG)
+        return fail(c, MCL_E_UNSUPPORTED, "inner solver %d not available on the device path yet", ctl->solver);
+    if (ctl->precision != MCL_PREC_FP64)
+        return fail(c, MCL_E_UNSUPPORTED, "precision %d not available yet", ctl->precision);
+    if (ctl->theta == 0.0) return fail(c, MCL_E_ARG, "theta = 0 divides by zero in the heat step");
+    c->ctl = *ctl;
+    c->have_ctl = true;
+    return 0;
+}
+
+static Level *level_of(mcl_ctx *c, int lev)
+{
+    for (auto &L : c->levels)
+        if (L.lev == lev) return &L;
+    return nullptr;
+}
+
+static int field_ptr(mcl_ctx *c, Level *L, int field, double **ptr, size_t *count)
+{
+    bool finest = L == &c->levels.back();
+    switch (field) {
+    case MCL_FIELD_T: *ptr = L->T; *count = L->nno; return 0;
+    case MCL_FIELD_VIS: *ptr = L->vis; *count = L->nno; return 0;
+    case MCL_FIELD_BUOYANCY: *ptr = L->buoy; *count = L->nno; return 0;
+    case MCL_FIELD_V: *ptr = L->V; *count = 2 * (size_t)L->nno; return 0;
+    case MCL_FIELD_D: *ptr = L->d; *count = L->neq; return 0;
+    case MCL_FIELD_P: *ptr = L->p; *count = L->nel; return 0;
+    case MCL_FIELD_F: *ptr = L->dev.F; *count = L->neq; return 0;
+    case MCL_FIELD_KDIAG: *ptr = L->dev.Kdiag; *count = L->neq; return 0;
+    case MCL_FIELD_STRESS: *ptr = L->stress; *count = L->nno; return 0;
+    }
+    if (!finest) return fail(c, MCL_E_ARG, "heat fields exist on the finest level only");
+    switch (field) {
+    case MCL_FIELD_HEAT_D: *ptr = c->hd; *count = L->neqT; return 0;
+    case MCL_FIELD_HEAT_DDOT: *ptr = c->hdd; *count = L->neqT; return 0;
+    case MCL_FIELD_HEAT_F: *ptr = c->hF; *count = L->neqT; return 0;
+    case MCL_FIELD_TAU: *ptr = c->tau; *count = 2 * (size_t)L->nel; return 0;
+    }
+    return fail(c, MCL_E_ARG, "unknown field %d", field);
+}
+
+extern "C" int mcl_set_field(mcl_ctx *c, int lev, int field, const double *src, size_t count)
+{
+    if (!c || !src) return MCL_E_ARG;
+    Level *L = level_of(c, lev);
+    if (!L) return fail(c, MCL_E_STATE, "level %d not uploaded", lev);
+    double *p;
+    size_t n;
+    int rc = field_ptr(c, L, field, &p, &n);
+    if (rc) return rc;
+    if (count != n) return fail(c, MCL_E_ARG, "field %d: %zu values, expected %zu", field, count, n);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(p, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (field == MCL_FIELD_HEAT_D) c->first_heat = false;
+    return 0;
+}
+
+extern "C" int mcl_get_field(mcl_ctx *c, int lev, int field, double *dst, size_t count)
+{
+    if (!c || !dst) return MCL_E_ARG;
+    Level *L = level_of(c, lev);
+    if (!L) return fail(c, MCL_E_STATE, "level %d not uploaded", lev);
+    double *p;
+    size_t n;
+    int rc = field_ptr(c, L, field, &p, &n);
+    if (rc) return rc;
+    if (count != n) return fail(c, MCL_E_ARG, "field %d: %zu values, expected %zu", field, count, n);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(dst, p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int mcl_get_matrix(mcl_ctx *c, int lev, int matrix, int width, int *ncol, int *col, double *val)
+{
+    if (!c || !ncol || !col || !val || width <= 0) return MCL_E_ARG;
+    Level *L = level_of(c, lev);
+    if (!L) return fail(c, MCL_E_STATE, "level %d not uploaded", lev);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int n, W;
+    const std::vector<int> *pn, *pc;
+    const double *dv;
+    bool finest = L == &c->levels.back();
+    if (matrix == MCL_MATRIX_K) { n = L->neq; W = L->WK; pn = &L->Kncol_h; pc = &L->Kcol_h; dv = L->dev.Kval; }
+    else if (matrix == MCL_MATRIX_HEAT_K || matrix == MCL_MATRIX_HEAT_C) {
+        if (!finest) return fail(c, MCL_E_ARG, "heat matrices exist on the finest level only");
+        n = L->neqT; W = L->WT; pn = &L->Tncol_h; pc = &L->Tcol_h;
+        dv = matrix == MCL_MATRIX_HEAT_K ? c->hK : c->hC;
+    } else if (matrix == MCL_MATRIX_G) {
+        // G is applied from element tables; rebuild its rows (columns = elements ascending)
+        std::vector<double> gt(MCL_MAX_LEVELS * 8);
+        HIPCHK(c, hipMemcpy(gt.data(), L->dev.gtab, gt.size() * sizeof(double), hipMemcpyDeviceToHost));
+        std::vector<int> cnt(L->nno), el(4 * (size_t)L->nno), aa(4 * (size_t)L->nno);
+        HIPCHK(c, hipMemcpy(cnt.data(), L->dev.p2n_cnt, cnt.size() * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(el.data(), L->dev.p2n_el, el.size() * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(aa.data(), L->dev.p2n_a, aa.size() * sizeof(int), hipMemcpyDeviceToHost));
+        for (int i = 0; i < L->nno; i++)
+            for (int d = 0; d < 2; d++) {
+                int P = L->id[d * L->nno + i];
+                if (P < 0) continue;
+                if (cnt[i] > width) return fail(c, MCL_E_ARG, "width too small");
+                ncol[P] = cnt[i];
+                for (int k = 0; k < cnt[i]; k++) {
+                    int e = el[k * (size_t)L->nno + i], a = aa[k * (size_t)L->nno + i];
+                    col[(size_t)P * width + k] = e;
+                    val[(size_t)P * width + k] = gt[(L->Mcode[e] & 15) * 8 + 2 * a + d];
+                }
+            }
+        return 0;
+    } else return fail(c, MCL_E_ARG, "unknown matrix %d", matrix);
+    if (W > width) return fail(c, MCL_E_ARG, "width %d < row length %d", width, W);
+    std::vector<double> v((size_t)std::max(W, 1) * std::max(n, 1));
+    HIPCHK(c, hipMemcpy(v.data(), dv, v.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) {
+        ncol[i] = (*pn)[i];
+        for (int s = 0; s < (*pn)[i]; s++) {
+            col[(size_t)i * width + s] = (*pc)[(size_t)s * n + i];
+            val[(size_t)i * width + s] = v[(size_t)s * n + i];
+        }
+    }
+    return 0;
+}
+
+extern "C" int mcl_set_model_time(mcl_ctx *c, double t)
+{
+    if (!c) return MCL_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(c->model_time, &t, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int mcl_get_model_time(mcl_ctx *c, double *t)
+{
+    if (!c || !t) return MCL_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(t, c->model_time, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int mcl_synchronize(mcl_ctx *c)
+{
+    if (!c) return MCL_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int ready(mcl_ctx *c)
+{
+    if (!c) return MCL_E_ARG;
+    if (c->nlev == 0) return fail(c, MCL_E_STATE, "no mesh uploaded");
+    if (!c->have_ctl) return fail(c, MCL_E_STATE, "no control set");
+    return 0;
+}
+
+static unsigned grid_for(mcl_ctx *c, long long n)
+{
+    long long b = (n + MCL_BLOCK - 1) / MCL_BLOCK;
+    b = std::max<long long>(1, std::min<long long>(b, 4LL * c->num_cu));
+    return (unsigned)b;
+}
+
+extern "C" int mcl_viscosity(mcl_ctx *c)
+{
+    int rc = ready(c);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    for (auto &L : c->levels) {
+        hipLaunchKernelGGL(k_viscosity, dim3(grid_for(c, L.nno)), dim3(MCL_BLOCK), 0, c->stream, L.nno, L.T,
+                           L.dev.X, c->ctl.A, c->ctl.refT, c->ctl.lm_jump, L.vis);
+        HIPCHK(c, hipGetLastError());
+    }
+    return 0;
+}
+
+extern "C" int mcl_buoyancy_force(mcl_ctx *c)
+{
+    int rc = ready(c);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    for (auto &L : c->levels) {
+        hipLaunchKernelGGL(k_buoy_rows, dim3((L.gnoz + MCL_BLOCK - 1) / MCL_BLOCK), dim3(MCL_BLOCK), 0, c->stream,
+                           L.gnoz, L.gnox, L.nno, L.brow_ptr, L.brow_e, L.brow_xi, L.brow_eta, L.dev.ienl, L.nel,
+                           L.T, c->ctl.Ra, L.avg);
+        HIPCHK(c, hipGetLastError());
+        hipLaunchKernelGGL(k_buoy_apply, dim3(grid_for(c, L.nno)), dim3(MCL_BLOCK), 0, c->stream, L.nno, L.gnoz,
+                           L.dev.node, L.T, c->ctl.Ra, L.avg, L.buoy);
+        HIPCHK(c, hipGetLastError());
+    }
+    return 0;
+}
+
+extern "C" int mcl_velocity_pressure_KGF_matrix(mcl_ctx *c)
+{
+    int rc = ready(c);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    for (auto &L : c->levels) {
+        if (L.neq == 0) continue;
+        hipLaunchKernelGGL(k_assemble_K, dim3(grid_for(c, L.neq)), dim3(MCL_BLOCK), 0, c->stream, L.dev,
+                           (const double *)L.vis, (const double *)L.buoy);
+        HIPCHK(c, hipGetLastError());
+    }
+    return 0;
+}
+
+static unsigned coop_blocks(mcl_ctx *c, long long n)
+{
+    long long b = (n + MCL_BLOCK - 1) / MCL_BLOCK;
+    b = std::max<long long>(1, std::min<long long>(b, c->max_coop_blocks));
+    return (unsigned)b;
+}
+
+static int reset_sync_if_aborted(mcl_ctx *c, const DevStats &st)
+{
+    if (st.aborted) {
+        HIPCHK(c, hipMemsetAsync(c->gs.count, 0, 3 * sizeof(unsigned), c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return fail(c, MCL_E_DEVICE, "grid barrier timed out");
+    }
+    return 0;
+}
+
+extern "C" int mcl_solve_velocity_pressure(mcl_ctx *c, mcl_stokes_stats *out)
+{
+    int rc = ready(c);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    Level &L = c->levels.back();
+    const mcl_control &k = c->ctl;
+    // initial_d_p (Solver.c:251-278): d = 0 on every level; p = 0 unless a
+    // previous call exists, AMR is off and zero_P is off.
+    if (!k.warm_start) {
+        for (auto &Lv : c->levels) {
+            HIPCHK(c, hipMemsetAsync(Lv.d, 0, std::max(Lv.neq, 1) * sizeof(double), c->stream));
+            if (!(c->been_here_dp && k.AMR == 0 && k.zero_P == 0))
+                HIPCHK(c, hipMemsetAsync(Lv.p, 0, Lv.nel * sizeof(double), c->stream));
+        }
+    }
+    c->been_here_dp = true;
+    HIPCHK(c, hipMemsetAsync(c->stats, 0, sizeof(DevStats), c->stream));
+    double accuracy = k.accuracy;
+    if (k.auto_accuracy) {
+        // Solver.c:41-51: accuracy = citcom_accu * sqrt(F.F / neq)
+        hipLaunchKernelGGL(k_residuals, dim3(grid_for(c, L.neq)), dim3(MCL_BLOCK), 0, c->stream, L.dev, c->rb,
+                           (const double *)L.d, (const double *)L.p);
+        hipLaunchKernelGGL(k_collect3, dim3(1), dim3(MCL_BLOCK), 0, c->stream, c->rb, 12,
+                           (int)grid_for(c, L.neq), c->resid);
+        double r[3];
+        HIPCHK(c, hipMemcpyAsync(r, c->resid, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        accuracy = k.citcom_accu * std::sqrt(r[2] / L.neq);
+    }
+    UzawaParams P{};
+    P.L = L.dev;
+    P.gs = c->gs;
+    P.rb = c->rb;
+    P.stats = c->stats;
+    P.d = L.d;
+    P.p = L.p;
+    size_t nv = std::max(L.neq, 1), np = L.nel;
+    double *w = c->work;
+    P.u1 = w; w += nv;
+    P.Ahv = w; w += nv;
+    P.cg_r = w; w += nv;
+    P.cg_s = w; w += nv;
+    P.cg_d = w; w += nv;
+    P.cg_q = w; w += nv;
+    P.Ahp = w; w += np;
+    P.BAB = w; w += np;
+    P.r0 = w; w += np;
+    P.r1 = w; w += np;
+    P.r2 = w; w += np;
+    P.z0 = w; w += np;
+    P.z1 = w; w += np;
+    P.s1 = w; w += np;
+    P.s2 = w; w += np;
+    P.accuracy = accuracy;
+    P.citcom_accu = k.citcom_accu;
+    P.max_citcom_cycle = k.max_citcom_cycle;
+    P.max_cg_cycle = k.max_cg_cycle;
+    P.cg_precondition = k.cg_precondition;
+    P.check_v_only = k.check_v_only;
+    P.schur = k.schur;
+    P.gs.nblocks = coop_blocks(c, std::max(L.neq, L.nel));
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    void *args[] = {&P};
+    HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_uzawa_fp64, dim3(P.gs.nblocks), dim3(MCL_BLOCK), args, 0,
+                                         c->stream));
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    // nodal velocity and stress (Solver.c:8-9)
+    for (auto &Lv : c->levels) {
+        if (&Lv != &L) continue;
+        hipLaunchKernelGGL(k_v_to_nodes, dim3(grid_for(c, Lv.nno)), dim3(MCL_BLOCK), 0, c->stream, Lv.dev,
+                           (const double *)Lv.d, Lv.V);
+        hipLaunchKernelGGL(k_stress_elem, dim3(grid_for(c, Lv.nel)), dim3(MCL_BLOCK), 0, c->stream, Lv.dev,
+                           (const double *)Lv.vis, (const double *)Lv.V, Lv.es, Lv.area);
+        hipLaunchKernelGGL(k_stress_node, dim3(grid_for(c, Lv.nno)), dim3(MCL_BLOCK), 0, c->stream, Lv.dev,
+                           (const double *)Lv.es, (const double *)Lv.area, Lv.stress);
+        HIPCHK(c, hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_residuals, dim3(grid_for(c, L.neq)), dim3(MCL_BLOCK), 0, c->stream, L.dev, c->rb,
+                       (const double *)L.d, (const double *)L.p);
+    hipLaunchKernelGGL(k_collect3, dim3(1), dim3(MCL_BLOCK), 0, c->stream, c->rb, 12, (int)grid_for(c, L.neq),
+                       c->resid);
+    HIPCHK(c, hipGetLastError());
+    DevStats st;
+    double r[3];
+    HIPCHK(c, hipMemcpyAsync(&st, c->stats, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(r, c->resid, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    rc = reset_sync_if_aborted(c, st);
+    if (rc) return rc;
+    if (out) {
+        out->uzawa_cycles = st.cycles;
+        out->dv = st.dv;
+        out->dp = st.dp;
+        out->converged = (st.dv <= k.citcom_accu && (k.check_v_only || st.dp <= k.citcom_accu)) ? 1 : 0;
+        out->inner_iterations = st.inner;
+        out->spmv = st.spmv;
+        out->accuracy_used = accuracy;
+        double fn = std::sqrt(r[2]);
+        out->res_momentum = fn > 0 ? std::sqrt(r[0]) / fn : std::sqrt(r[0]);
+        out->res_continuity = fn > 0 ? std::sqrt(r[1]) / fn : std::sqrt(r[1]);
+        out->device_ms = ms;
+    }
+    if (st.nan_flag) return fail(c, MCL_E_NUMERIC, "NaN in the Stokes solve");
+    return 0;
+}
+
+extern "C" int mcl_update_temperature(mcl_ctx *c, mcl_heat_stats *out)
+{
+    int rc = ready(c);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    Level &L = c->levels.back();
+    const mcl_control &k = c->ctl;
+    HIPCHK(c, hipMemsetAsync(c->stats, 0, sizeof(DevStats), c->stream));
+    HeatParams H{};
+    H.L = L.dev;
+    H.gs = c->gs;
+    H.rb = c->rb;
+    H.stats = c->stats;
+    H.V = L.V;
+    H.T = L.T;
+    H.tau = c->tau;
+    H.hK = c->hK;
+    H.hC = c->hC;
+    H.hA = c->hA;
+    H.hF = c->hF;
+    H.hd = c->hd;
+    H.hdd = c->hdd;
+    H.d0 = c->hd0;
+    H.B = c->hB;
+    double *w = c->work;
+    size_t nt = std::max(L.neqT, 1);
+    H.cg_r = w; w += nt;
+    H.cg_s = w; w += nt;
+    H.cg_d = w; w += nt;
+    H.cg_q = w; w += nt;
+    H.model_time = c->model_time;
+    H.accuracy = k.accuracy;
+    H.theta = k.theta;
+    H.Q = k.Q;
+    H.Llast0 = L.Llast0;
+    H.Llast1 = L.Llast1;
+    H.diff_dt = L.diff_dt;
+    H.first_call = c->first_heat ? 1 : 0;
+    H.cg_precondition = k.cg_precondition;
+    H.gs.nblocks = coop_blocks(c, std::max(L.neqT, L.nel));
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    void *args[] = {&H};
+    HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_heat_fp64, dim3(H.gs.nblocks), dim3(MCL_BLOCK), args, 0,
+                                         c->stream));
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    DevStats st;
+    double mt;
+    HIPCHK(c, hipMemcpyAsync(&st, c->stats, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&mt, c->model_time, sizeof mt, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    rc = reset_sync_if_aborted(c, st);
+    if (rc) return rc;
+    c->first_heat = false;
+    if (out) {
+        out->cg_cycles = st.cycles;
+        out->init_cg_cycles = st.init_cycles;
+        out->dt = st.dt;
+        out->model_time = mt;
+        out->device_ms = ms;
+    }
+    return 0;
+}

@@ -1,0 +1,198 @@
+// Device utilities shared by the gfx950 kernels of libmcl_hip.so:
+//  * a grid-wide barrier for persistent (cooperatively launched) kernels,
+//  * compensated FP64 reductions (double-double accumulation, exact products),
+//  * the bilinear-quad shape tables (Shape_function.c:17-121 values).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MCL_BLOCK 256
+#define MCL_WAVE 64
+#define MCL_MAX_LEVELS 20
+#define MCL_MAX_WG 2048
+#define MCL_SPIN_LIMIT (1u << 24)
+
+// Shape tables, uploaded once from the host (computed there with the
+// reference's expressions so the values are identical bit for bit).
+struct ShapeTables {
+    double N[4][4];      // N[a][g] at Gauss point g
+    double dxi[4][4];    // dN_a/dxi at g
+    double deta[4][4];   // dN_a/deta at g
+    double dxic[4], detac[4];
+};
+
+// Per-level element sizes (G.dx, G.dz) passed by value to kernels.
+struct LevelSizes {
+    double dx[MCL_MAX_LEVELS];
+    double dz[MCL_MAX_LEVELS];
+};
+
+// ---------------------------------------------------------------- grid barrier
+// Sense/generation barrier.  Thread 0 of every workgroup arrives with an
+// agent-scope release, the last arrival resets the counter and bumps the
+// generation, the others spin on the generation with agent-scope acquires.
+// The spin is bounded: past MCL_SPIN_LIMIT polls the barrier sets *abort and
+// returns, and every caller then leaves its loops (all waves reach the end).
+struct GridSync {
+    unsigned *count;
+    unsigned *gen;
+    unsigned *abort;
+    unsigned nblocks;
+};
+
+__device__ __forceinline__ bool grid_sync(const GridSync &gs, unsigned &my_gen)
+{
+    __shared__ unsigned s_abort;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned g = my_gen;
+        __threadfence();
+        unsigned arrived = __hip_atomic_fetch_add(gs.count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == gs.nblocks - 1) {
+            __hip_atomic_store(gs.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(gs.gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            unsigned spins = 0;
+            while (__hip_atomic_load(gs.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > MCL_SPIN_LIMIT) {
+                    __hip_atomic_store(gs.abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                if ((spins & 1023u) == 0 &&
+                    __hip_atomic_load(gs.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    break;
+            }
+        }
+        __threadfence();
+        my_gen = g + 1;
+        s_abort = __hip_atomic_load(gs.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    return s_abort == 0;
+}
+
+// ------------------------------------------------------- compensated reduction
+// A double-double value (hi + lo).  Sums use TwoSum, products TwoProduct via
+// fma, so a dot product is accumulated as if in ~106-bit precision and rounded
+// once at the end (Ogita, Rump, Oishi 2005, "Accurate sum and dot product").
+struct dd {
+    double hi, lo;
+};
+
+__device__ __forceinline__ dd dd_zero() { return dd{0.0, 0.0}; }
+
+__device__ __forceinline__ dd two_sum(double a, double b)
+{
+    double s = a + b;
+    double bb = s - a;
+    double e = (a - (s - bb)) + (b - bb);
+    return dd{s, e};
+}
+
+__device__ __forceinline__ dd dd_add(dd a, dd b)
+{
+    dd s = two_sum(a.hi, b.hi);
+    double lo = s.lo + (a.lo + b.lo);
+    double hi = s.hi + lo;
+    return dd{hi, lo - (hi - s.hi)};
+}
+
+__device__ __forceinline__ dd dd_add_prod(dd acc, double x, double y)
+{
+    double p = x * y;
+    double pe = __builtin_fma(x, y, -p);
+    dd s = two_sum(acc.hi, p);
+    double lo = s.lo + (acc.lo + pe);
+    double hi = s.hi + lo;
+    return dd{hi, lo - (hi - s.hi)};
+}
+
+__device__ __forceinline__ dd dd_shfl_xor(dd v, int mask)
+{
+    return dd{__shfl_xor(v.hi, mask, MCL_WAVE), __shfl_xor(v.lo, mask, MCL_WAVE)};
+}
+
+__device__ __forceinline__ double dmax_shfl(double v)
+{
+    for (int m = MCL_WAVE / 2; m > 0; m >>= 1) v = fmax(v, __shfl_xor(v, m, MCL_WAVE));
+    return v;
+}
+
+// Block reduction of NR double-double values (fixed order, deterministic).
+template <int NR>
+__device__ __forceinline__ void block_reduce_dd(dd (&v)[NR], dd (*lds)[MCL_BLOCK / MCL_WAVE])
+{
+    const int lane = threadIdx.x & (MCL_WAVE - 1), wave = threadIdx.x / MCL_WAVE;
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+        for (int m = MCL_WAVE / 2; m > 0; m >>= 1) v[r] = dd_add(v[r], dd_shfl_xor(v[r], m));
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int r = 0; r < NR; r++) lds[r][wave] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        dd t = lds[r][0];
+        for (int w = 1; w < MCL_BLOCK / MCL_WAVE; w++) t = dd_add(t, lds[r][w]);
+        v[r] = t;
+    }
+}
+
+// Partial-sum exchange through global memory: every workgroup writes its
+// partials, and after a grid barrier every workgroup reduces all of them in the
+// same order, so every workgroup obtains the identical value.
+struct RedBuf {
+    dd *part;      // [slot][MCL_MAX_WG]
+    double *pmax;  // [slot][MCL_MAX_WG]
+};
+
+template <int NR>
+__device__ __forceinline__ void reduce_publish(const RedBuf &rb, int slot0, dd (&v)[NR],
+                                               dd (*lds)[MCL_BLOCK / MCL_WAVE])
+{
+    block_reduce_dd<NR>(v, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int r = 0; r < NR; r++) rb.part[(slot0 + r) * MCL_MAX_WG + blockIdx.x] = v[r];
+}
+
+template <int NR>
+__device__ __forceinline__ void reduce_collect(const RedBuf &rb, int slot0, int nblocks, double (&out)[NR],
+                                               dd (*lds)[MCL_BLOCK / MCL_WAVE])
+{
+    dd v[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) v[r] = dd_zero();
+    for (int b = threadIdx.x; b < nblocks; b += MCL_BLOCK)
+#pragma unroll
+        for (int r = 0; r < NR; r++) v[r] = dd_add(v[r], rb.part[(slot0 + r) * MCL_MAX_WG + b]);
+    block_reduce_dd<NR>(v, lds);
+#pragma unroll
+    for (int r = 0; r < NR; r++) out[r] = v[r].hi + v[r].lo;
+}
+
+__device__ __forceinline__ double block_max(double v, double *lds)
+{
+    const int lane = threadIdx.x & (MCL_WAVE - 1), wave = threadIdx.x / MCL_WAVE;
+    v = dmax_shfl(v);
+    __syncthreads();
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    double t = lds[0];
+    for (int w = 1; w < MCL_BLOCK / MCL_WAVE; w++) t = fmax(t, lds[w]);
+    return t;
+}
+
+__device__ __forceinline__ double block_min(double v, double *lds)
+{
+    const int lane = threadIdx.x & (MCL_WAVE - 1), wave = threadIdx.x / MCL_WAVE;
+    for (int m = MCL_WAVE / 2; m > 0; m >>= 1) v = fmin(v, __shfl_xor(v, m, MCL_WAVE));
+    __syncthreads();
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    double t = lds[0];
+    for (int w = 1; w < MCL_BLOCK / MCL_WAVE; w++) t = fmin(t, lds[w]);
+    return t;
+}

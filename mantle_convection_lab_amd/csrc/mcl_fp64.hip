@@ -1,0 +1,818 @@
+// FP64 kernels of libmcl_hip.so (gfx950).
+//
+// This translation unit is compiled with -ffp-contract=off: every operation
+// on the parity path rounds like the reference's x86-64 build (no FMA), and
+// every sum runs in the reference's order (element-ascending assembly, sorted
+// row products), so assembly, sparse products and vector updates reproduce the
+// reference bit for bit.  Only the dot products differ: they are compensated
+// parallel reductions (the reference sums sequentially).
+//
+// Work decomposition (one HIP thread per item, grid-stride):
+//   nodes     viscosity, buoyancy, nodal velocity, stress averaging
+//   rows      velocity / heat equations: assembly of one matrix row from the
+//             row's node-adjacent elements (atomic-free, element-ascending),
+//             ELL products, G p, CG vector updates
+//   elements  BAB, G^T d, pressure-space updates, SUPG tau, element stress
+// The Uzawa solve and the heat step each run as one persistent kernel launched
+// cooperatively; phases are separated by grid barriers (mcl_device.h).
+#include "mcl_kernels.h"
+
+__constant__ ShapeTables c_shape;
+
+extern "C" hipError_t mcl_upload_shape(const ShapeTables *s)
+{
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_shape), s, sizeof(ShapeTables));
+}
+
+// ------------------------------------------------------------------ physics
+
+// Viscosity.c:3-24: eta = exp(A (refT - T)), times lm_jump below z = 0.77.
+__global__ void k_viscosity(int nno, const double *__restrict__ T, const double *__restrict__ X,
+                            double A, double refT, double lm_jump, double *__restrict__ vis)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nno; i += gridDim.x * blockDim.x) {
+        double v = exp(A * (refT - T[i]));
+        if (X[nno + i] < 0.77) v *= lm_jump;
+        vis[i] = v;
+    }
+}
+
+// Buoyancy.c:3-22 with get_node_horiz_avg (Parallel.c:354-437): one thread per
+// node row k walks the row's contribution list in the reference's order
+// (elements ascending; within an element the interior points, then the top
+// row or right column points, then the corner node).  Each entry adds
+// N_a(xi, eta) * data[node_a] for a = 0..3 separately, as the reference does.
+__global__ void k_buoy_rows(int gnoz, int gnox, int nno, const int *__restrict__ ptr,
+                            const int *__restrict__ ent_e, const signed char *__restrict__ ent_xi,
+                            const signed char *__restrict__ ent_eta, const int *__restrict__ ienl, int nel,
+                            const double *__restrict__ T, double Ra, double *__restrict__ avg)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= gnoz) return;
+    double s = 0.0;
+    for (int t = ptr[k]; t < ptr[k + 1]; t++) {
+        int e = ent_e[t];
+        if (e < 0) {
+            s += Ra * T[nno - 1];
+            continue;
+        }
+        double xi = (double)ent_xi[t], eta = (double)ent_eta[t];
+        // Na values of Shape_function.c:180-197
+        double n0 = 0.25 * (1.0 - xi) * (1.0 - eta);
+        double n1 = 0.25 * (1.0 - xi) * (1.0 + eta);
+        double n2 = 0.25 * (1.0 + xi) * (1.0 + eta);
+        double n3 = 0.25 * (1.0 + xi) * (1.0 - eta);
+        s += n0 * (Ra * T[ienl[e]]);
+        s += n1 * (Ra * T[ienl[nel + e]]);
+        s += n2 * (Ra * T[ienl[2 * nel + e]]);
+        s += n3 * (Ra * T[ienl[3 * nel + e]]);
+    }
+    avg[k] = s / gnox;
+}
+
+__global__ void k_buoy_apply(int nno, int gnoz, const int *__restrict__ node, const double *__restrict__ T,
+                             double Ra, const double *__restrict__ avg, double *__restrict__ buoy)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nno; i += gridDim.x * blockDim.x) {
+        int nd = node[i];
+        int ix = nd / gnoz, iz = nd - ix * gnoz;
+        buoy[i] = Ra * T[i] - avg[iz];
+    }
+}
+
+// ------------------------------------------------------------ Stokes assembly
+
+// Upper-triangle element stiffness entry of get_element_K (Build_matrix.c:
+// 192-271) for row dof (a,i) and column dof (b,j) with 2a+i <= 2b+j; eta_g is
+// the linearly interpolated viscosity at the four Gauss points.
+__device__ __forceinline__ double ke_upper(int a, int i, int b, int j, const double eta_g[4], double dxdxi,
+                                           double dzdeta, double jac)
+{
+    const double dzdxi = 0.0, dxdeta = 0.0;
+    double K = 0.0;
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        double eta = eta_g[g];
+        double D00 = 2.0 * eta, D11 = 2.0 * eta, D22 = eta;
+        const double D01 = 0.0, D02 = 0.0, D10 = 0.0, D12 = 0.0, D20 = 0.0, D21 = 0.0;
+        double ax = c_shape.dxi[a][g] * dzdeta - c_shape.deta[a][g] * dzdxi;
+        double az = c_shape.deta[a][g] * dxdxi - c_shape.dxi[a][g] * dxdeta;
+        double bx = c_shape.dxi[b][g] * dzdeta - c_shape.deta[b][g] * dzdxi;
+        double bz = c_shape.deta[b][g] * dxdxi - c_shape.dxi[b][g] * dxdeta;
+        if (i == 0 && j == 0)
+            K += ((ax * D00 + az * D20) * bx + (ax * D02 + az * D22) * bz);
+        else if (i == 1 && j == 0)
+            K += ((az * D10 + ax * D20) * bx + (az * D12 + ax * D22) * bz);
+        else if (i == 0 && j == 1)
+            K += ((ax * D01 + az * D21) * bz + (ax * D02 + az * D22) * bx);
+        else
+            K += ((az * D11 + ax * D21) * bz + (az * D12 + ax * D22) * bx);
+    }
+    return K / jac;
+}
+
+// Element entry K_e[p][q] after the reference's mirror step: the value of the
+// upper entry (min(p,q), max(p,q)).
+__device__ __forceinline__ double ke_entry(int p, int q, const double eta_g[4], double dxdxi, double dzdeta,
+                                           double jac)
+{
+    int lo = p < q ? p : q, hi = p < q ? q : p;
+    return ke_upper(lo >> 1, lo & 1, hi >> 1, hi & 1, eta_g, dxdxi, dzdeta, jac);
+}
+
+__device__ __forceinline__ void elem_eta(const LevelDev &L, int e, const double *__restrict__ vis, double eta_g[4])
+{
+    double vn[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++) vn[d] = vis[L.ienl[d * L.nel + e]];
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        double s = 0.0;
+#pragma unroll
+        for (int d = 0; d < 4; d++) s += c_shape.N[d][g] * vn[d];
+        eta_g[g] = s;
+    }
+}
+
+// One thread per velocity row P: K row slots (element-ascending sums, the
+// order of K_matrix, Build_matrix.c:42-115), F[P] (all boundary-value
+// subtractions first, then the element loads of F_matrix, :117-140), Kdiag.
+// The thread owns its row, so the slots accumulate in place without atomics.
+__global__ void k_assemble_K(LevelDev L, const double *__restrict__ vis, const double *__restrict__ buoy)
+{
+    for (int P = blockIdx.x * blockDim.x + threadIdx.x; P < L.neq; P += gridDim.x * blockDim.x) {
+        const int n = L.veq_node[P], ip = L.veq_dof[P];
+        const int nadj = L.p2n_cnt[n];
+        const int nc = L.Kncol[P];
+        for (int s = 0; s < L.WK; s++) L.Kval[s * L.neq + P] = 0.0;
+        double F = 0.0;
+        for (int k = 0; k < nadj; k++) {
+            int e = L.p2n_el[k * L.nno + n], a = L.p2n_a[k * L.nno + n];
+            int lev = L.elev[e];
+            double dxdxi = 0.5 * L.lvl_dx[lev], dzdeta = 0.5 * L.lvl_dz[lev];
+            double jac = dxdxi * dzdeta - 0.0 * 0.0;
+            double eta_g[4];
+            elem_eta(L, e, vis, eta_g);
+            int p = 2 * a + ip;
+            for (int j = 0; j < 2; j++)
+                for (int b = 0; b < 4; b++) {
+                    int q = 2 * b + j;
+                    double kv = ke_entry(p, q, eta_g, dxdxi, dzdeta, jac);
+                    int slot = L.Kslot[(k * 8 + q) * L.neq + P];
+                    if (slot >= 0) L.Kval[slot * L.neq + P] += kv;
+                    int nb = L.ienl[b * L.nel + e];
+                    if (L.v_bc[j * L.nno + nb] == 'V') F -= kv * L.v_bc_val[j * L.nno + nb];
+                }
+        }
+        // element loads, get_element_F (Build_matrix.c:273-344)
+        for (int k = 0; k < nadj; k++) {
+            int e = L.p2n_el[k * L.nno + n], a = L.p2n_a[k * L.nno + n];
+            int lev = L.elev[e];
+            double dxdxi = 0.5 * L.lvl_dx[lev], dzdeta = 0.5 * L.lvl_dz[lev];
+            double jac = dxdxi * dzdeta - 0.0 * 0.0;
+            int idx[4];
+#pragma unroll
+            for (int d = 0; d < 4; d++) idx[d] = L.ienl[d * L.nel + e];
+            double Fe = 0.0;
+            if (ip == 1) {
+                double fg[4];
+#pragma unroll
+                for (int g = 0; g < 4; g++) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int d = 0; d < 4; d++) s += c_shape.N[d][g] * buoy[idx[d]];
+                    fg[g] = s;
+                }
+#pragma unroll
+                for (int g = 0; g < 4; g++) Fe += c_shape.N[a][g] * fg[g];
+                Fe *= jac;
+            }
+            int nat = 0;
+#pragma unroll
+            for (int d = 0; d < 4; d++)
+                if (L.v_bc[ip * L.nno + idx[d]] == 'S') nat += d;
+            if (nat == 3) nat = 1;
+            else if (nat == 1 || nat == 5) nat = 2;
+            else if (nat == 0 || nat == 2) nat = 0;
+            if (nat != 0) {
+                double Lx = L.X[ip * L.nno + idx[2]] - L.X[ip * L.nno + idx[0]];
+                Fe += 0.5 * Lx * L.v_bc_val[ip * L.nno + idx[a]];
+            }
+            F += Fe;
+        }
+        double diag = 0.0;
+        for (int s = 0; s < nc; s++)
+            if (L.Kcol[s * L.neq + P] == P) diag = L.Kval[s * L.neq + P];
+        L.F[P] = F;
+        L.Kdiag[P] = diag;
+    }
+}
+
+// ------------------------------------------------------------- sparse products
+
+__device__ __forceinline__ double ell_row(const int *__restrict__ col, const double *__restrict__ val, int ncol,
+                                          int n, int row, const double *__restrict__ x)
+{
+    double s = 0.0;
+    for (int j = 0; j < ncol; j++) s += val[j * n + row] * x[col[j * n + row]];
+    return s;
+}
+
+// (G p)[P], G_prod_p (Solver.c:569-591): columns are the row's elements, ascending.
+__device__ __forceinline__ double g_row(const LevelDev &L, int P, const double *__restrict__ p)
+{
+    int n = L.veq_node[P], ip = L.veq_dof[P];
+    double s = 0.0;
+    for (int k = 0; k < L.p2n_cnt[n]; k++) {
+        int e = L.p2n_el[k * L.nno + n], a = L.p2n_a[k * L.nno + n];
+        s += L.gtab[L.elev[e] * 8 + 2 * a + ip] * p[e];
+    }
+    return s;
+}
+
+// (G^T x)[e], GT_prod_d (Solver.c:645-666): rows of the element ascending.
+__device__ __forceinline__ double gt_elem(const LevelDev &L, int e, const double *__restrict__ x)
+{
+    double s = 0.0;
+    int lev = L.elev[e];
+    for (int t = 0; t < 8; t++) {
+        int r = L.gt_row[t * L.nel + e];
+        if (r < 0) break;
+        s += L.gtab[lev * 8 + L.gt_p[t * L.nel + e]] * x[r];
+    }
+    return s;
+}
+
+// ------------------------------------------------------------ persistent Uzawa
+
+struct CgVecs {
+    double *r, *s, *d, *q;
+};
+
+// Hanging-dof constraint, enforce_constrain_veq (Solver.c:1368-1437).
+__device__ void constrain_v(const LevelDev &L, double *__restrict__ x, int gtid, int gsize)
+{
+    if (L.nh == 0) return;
+    if (L.h_serial) {
+        if (gtid != 0) return;
+        for (int k = 0; k < L.nh; k++) {
+            for (int d = 0; d < 2; d++) {
+                double t = 0.0;
+                for (int c = 0; c < 2; c++) {
+                    int kn = L.h_ci[c * L.nh + k];
+                    int eqc = L.id[d * L.nno + kn];
+                    t += eqc != -1 ? x[eqc] : L.v_bc_val[d * L.nno + kn];
+                }
+                t *= 0.5;
+                int eq = L.id[d * L.nno + L.h_ni[k]];
+                if (eq >= 0) x[eq] = t;
+            }
+        }
+        return;
+    }
+    for (int k = gtid; k < L.nh; k += gsize) {
+        for (int d = 0; d < 2; d++) {
+            double t = 0.0;
+            for (int c = 0; c < 2; c++) {
+                int kn = L.h_ci[c * L.nh + k];
+                int eqc = L.id[d * L.nno + kn];
+                t += eqc != -1 ? x[eqc] : L.v_bc_val[d * L.nno + kn];
+            }
+            t *= 0.5;
+            int eq = L.id[d * L.nno + L.h_ni[k]];
+            if (eq >= 0) x[eq] = t;
+        }
+    }
+}
+
+__device__ void constrain_T(const LevelDev &L, double *__restrict__ x, int gtid, int gsize)
+{
+    if (L.nh == 0) return;
+    int start = L.h_serial ? (gtid == 0 ? 0 : L.nh) : gtid;
+    int step = L.h_serial ? 1 : gsize;
+    for (int k = start; k < L.nh; k += step) {
+        double t = 0.0;
+        for (int c = 0; c < 2; c++) {
+            int kn = L.h_ci[c * L.nh + k];
+            int eqc = L.heat_id[kn];
+            t += eqc != -1 ? x[eqc] : L.T_bc_val[kn];
+        }
+        t *= 0.5;
+        int eq = L.heat_id[L.h_ni[k]];
+        if (eq >= 0) x[eq] = t;
+    }
+}
+
+// An ELL matrix with its diagonal (explicit array or slot index per row).
+struct EllMat {
+    const int *col;
+    const int *ncol;
+    const double *val;
+    const double *diag;      // diagonal values, or NULL to use diag_slot
+    const int *diag_slot;
+    int n;
+};
+
+__device__ __forceinline__ double ell_diag(const EllMat &A, int i)
+{
+    return A.diag ? A.diag[i] : A.val[A.diag_slot[i] * A.n + i];
+}
+
+// Conj_Grad (Solver.c:1589-1797) / Conj_Grad_heat (Update_temperature.c:
+// 1140-1354) inside a persistent kernel.  All workgroups execute the same
+// control flow: the scalars come from reduce_collect, identical everywhere.
+// kind 0: velocity (constraint on velocity dofs), 1: heat.
+// Returns the iteration count; *ok is cleared if a barrier aborted.
+__device__ int pcg_persistent(const LevelDev &L, const EllMat &A, const double *__restrict__ b,
+                              double *__restrict__ x, bool x_zero, double accu, int max_cycle, int precond,
+                              int kind, CgVecs v, const GridSync &gs, unsigned &gen, const RedBuf &rb,
+                              dd (*lds)[MCL_BLOCK / MCL_WAVE], double *ldsm, bool &ok, long long &spmv)
+{
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsize = gridDim.x * blockDim.x;
+    const int n = A.n;
+    {
+        dd acc[1] = {dd_zero()};
+        for (int i = gtid; i < n; i += gsize) {
+            double ax = x_zero ? 0.0 : ell_row(A.col, A.val, A.ncol[i], n, i, x);
+            double r = b[i] - ax;
+            double dv = precond ? r / ell_diag(A, i) : r;
+            v.r[i] = r;
+            v.d[i] = dv;
+            acc[0] = dd_add_prod(acc[0], r, dv);
+        }
+        if (!x_zero) spmv++;
+        reduce_publish<1>(rb, 0, acc, lds);
+    }
+    ok = ok && grid_sync(gs, gen);
+    double delta_new;
+    {
+        double o[1];
+        reduce_collect<1>(rb, 0, gridDim.x, o, lds);
+        delta_new = o[0];
+    }
+    const double delta0 = delta_new;
+    int cycle = 0;
+    while (ok && cycle < max_cycle && delta_new > accu * delta0) {
+        // q = K d, d.q
+        {
+            dd acc[1] = {dd_zero()};
+            for (int i = gtid; i < n; i += gsize) {
+                double q = ell_row(A.col, A.val, A.ncol[i], n, i, v.d);
+                v.q[i] = q;
+                acc[0] = dd_add_prod(acc[0], v.d[i], q);
+            }
+            spmv++;
+            reduce_publish<1>(rb, 1, acc, lds);
+        }
+        ok = ok && grid_sync(gs, gen);
+        double t1;
+        {
+            double o[1];
+            reduce_collect<1>(rb, 1, gridDim.x, o, lds);
+            t1 = o[0];
+        }
+        const double alpha = delta_new / t1;
+        for (int i = gtid; i < n; i += gsize) x[i] += alpha * v.d[i];
+        if (L.nh > 0) {
+            ok = ok && grid_sync(gs, gen);
+            if (kind == 0) constrain_v(L, x, gtid, gsize);
+            else constrain_T(L, x, gtid, gsize);
+        }
+        const bool recompute = (cycle % 50) == 0;
+        if (recompute) ok = ok && grid_sync(gs, gen);
+        double delta_old = delta_new;
+        {
+            dd acc[1] = {dd_zero()};
+            double mx = 0.0;
+            for (int i = gtid; i < n; i += gsize) {
+                double r;
+                if (recompute) r = b[i] - ell_row(A.col, A.val, A.ncol[i], n, i, x);
+                else r = v.r[i] - alpha * v.q[i];
+                v.r[i] = r;
+                double s = precond ? r / ell_diag(A, i) : r;
+                v.s[i] = s;
+                acc[0] = dd_add_prod(acc[0], r, s);
+                mx = fmax(mx, fabs(alpha * r));
+            }
+            if (recompute) spmv++;
+            reduce_publish<1>(rb, 2, acc, lds);
+            double bm = block_max(mx, ldsm);
+            if (threadIdx.x == 0) rb.pmax[0 * MCL_MAX_WG + blockIdx.x] = bm;
+        }
+        ok = ok && grid_sync(gs, gen);
+        double res;
+        {
+            double o[1];
+            reduce_collect<1>(rb, 2, gridDim.x, o, lds);
+            delta_new = o[0];
+            double m = 0.0;
+            for (int bb = threadIdx.x; bb < (int)gridDim.x; bb += MCL_BLOCK) m = fmax(m, rb.pmax[bb]);
+            res = block_max(m, ldsm);
+        }
+        const double beta = delta_new / delta_old;
+        for (int i = gtid; i < n; i += gsize) v.d[i] = v.s[i] + beta * v.d[i];
+        cycle++;
+        bool brk = precond ? (res < accu) : (kind == 0 ? res < accu : (res < accu && cycle > 10));
+        ok = ok && grid_sync(gs, gen);
+        if (brk) break;
+    }
+    return cycle;
+}
+
+// citcom_solver (Solver.c:14-249) as one persistent kernel.
+__global__ void __launch_bounds__(MCL_BLOCK) k_uzawa_fp64(UzawaParams P)
+{
+    __shared__ dd lds[4][MCL_BLOCK / MCL_WAVE];
+    __shared__ double ldsm[MCL_BLOCK / MCL_WAVE];
+    const LevelDev &L = P.L;
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsize = gridDim.x * blockDim.x;
+    const int neq = L.neq, nel = L.nel;
+    unsigned gen = __hip_atomic_load(P.gs.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    bool ok = true;
+    long long spmv = 0, inner = 0;
+    EllMat K{L.Kcol, L.Kncol, L.Kval, L.Kdiag, nullptr, neq};
+    CgVecs cv{P.cg_r, P.cg_s, P.cg_d, P.cg_q};
+    // Preconditioner data, Solver.c:54-64: BAB[e] = sum_i G_ie^2 / Kdiag_i
+    for (int e = gtid; e < nel; e += gsize) {
+        double s = 0.0;
+        int lev = L.elev[e];
+        for (int t = 0; t < 8; t++) {
+            int r = L.gt_row[t * nel + e];
+            if (r < 0) break;
+            double g = L.gtab[lev * 8 + L.gt_p[t * nel + e]];
+            s += g * g / L.Kdiag[r];
+        }
+        P.BAB[e] = s;
+    }
+    // u1 = K d ; Ah = F - G p - u1   (:67-77)
+    for (int i = gtid; i < neq; i += gsize) {
+        double u = ell_row(L.Kcol, L.Kval, L.Kncol[i], neq, i, P.d);
+        double gp = g_row(L, i, P.p);
+        P.u1[i] = u;
+        P.Ahv[i] = L.F[i] - gp - u;
+    }
+    spmv++;
+    ok = ok && grid_sync(P.gs, gen);
+    inner += pcg_persistent(L, K, P.Ahv, P.u1, false, P.accuracy, P.max_cg_cycle, P.cg_precondition, 0, cv,
+                            P.gs, gen, P.rb, lds, ldsm, ok, spmv);
+    for (int i = gtid; i < neq; i += gsize) P.d[i] += P.u1[i];
+    ok = ok && grid_sync(P.gs, gen);
+    double *r0 = P.r0, *r1 = P.r1, *r2 = P.r2, *z0 = P.z0, *z1 = P.z1, *s1 = P.s1, *s2 = P.s2;
+    for (int e = gtid; e < nel; e += gsize) r1[e] = gt_elem(L, e, P.d);
+    double dv = 1.0, dp = 1.0;
+    int cycle = 0;
+    while (ok && (dv > P.citcom_accu || (dp > P.citcom_accu && P.check_v_only == 0)) &&
+           cycle < P.max_citcom_cycle) {
+        // z1 = BAB r1 (or r1 / BAB); r1.z1, r0.z0
+        {
+            dd acc[2] = {dd_zero(), dd_zero()};
+            for (int e = gtid; e < nel; e += gsize) {
+                double z = P.schur ? r1[e] / P.BAB[e] : P.BAB[e] * r1[e];
+                z1[e] = z;
+                acc[0] = dd_add_prod(acc[0], r1[e], z);
+                if (cycle > 0) acc[1] = dd_add_prod(acc[1], r0[e], z0[e]);
+            }
+            reduce_publish<2>(P.rb, 4, acc, lds);
+        }
+        ok = ok && grid_sync(P.gs, gen);
+        double r1z1, delta = 0.0;
+        {
+            double o[2];
+            reduce_collect<2>(P.rb, 4, gridDim.x, o, lds);
+            r1z1 = o[0];
+            if (cycle > 0) delta = r1z1 / o[1];
+        }
+        if (!(r1z1 == r1z1)) P.stats->nan_flag = 1;
+        for (int e = gtid; e < nel; e += gsize) s2[e] = cycle == 0 ? z1[e] : z1[e] + delta * s1[e];
+        ok = ok && grid_sync(P.gs, gen);
+        // Ah = G s2 ; u1 = 0 ; K u1 = Ah
+        for (int i = gtid; i < neq; i += gsize) {
+            P.Ahv[i] = g_row(L, i, s2);
+            P.u1[i] = 0.0;
+        }
+        ok = ok && grid_sync(P.gs, gen);
+        inner += pcg_persistent(L, K, P.Ahv, P.u1, true, P.accuracy, P.max_cg_cycle, P.cg_precondition, 0, cv,
+                                P.gs, gen, P.rb, lds, ldsm, ok, spmv);
+        // Ah = G^T u1 ; alpha = r1.z1 / s2.Ah
+        {
+            dd acc[1] = {dd_zero()};
+            for (int e = gtid; e < nel; e += gsize) {
+                double a = gt_elem(L, e, P.u1);
+                P.Ahp[e] = a;
+                acc[0] = dd_add_prod(acc[0], s2[e], a);
+            }
+            reduce_publish<1>(P.rb, 6, acc, lds);
+        }
+        ok = ok && grid_sync(P.gs, gen);
+        double alpha;
+        {
+            double o[1];
+            reduce_collect<1>(P.rb, 6, gridDim.x, o, lds);
+            alpha = r1z1 / o[0];
+        }
+        {
+            dd acc[4] = {dd_zero(), dd_zero(), dd_zero(), dd_zero()};
+            for (int e = gtid; e < nel; e += gsize) {
+                r2[e] = r1[e] - alpha * P.Ahp[e];
+                double pe = P.p[e] + alpha * s2[e];
+                P.p[e] = pe;
+                acc[0] = dd_add_prod(acc[0], s2[e], s2[e]);
+                acc[1] = dd_add_prod(acc[1], pe, pe);
+            }
+            for (int i = gtid; i < neq; i += gsize) {
+                double u = P.u1[i];
+                double di = P.d[i] - alpha * u;
+                P.d[i] = di;
+                acc[2] = dd_add_prod(acc[2], u, u);
+                acc[3] = dd_add_prod(acc[3], di, di);
+            }
+            reduce_publish<4>(P.rb, 8, acc, lds);
+        }
+        ok = ok && grid_sync(P.gs, gen);
+        {
+            double o[4];
+            reduce_collect<4>(P.rb, 8, gridDim.x, o, lds);
+            dp = alpha * sqrt(o[0] / (1.0e-32 + o[1]));
+            dv = alpha * sqrt(o[2] / (1.0e-32 + o[3]));
+        }
+        cycle++;
+        double *sh;
+        sh = s1; s1 = s2; s2 = sh;
+        sh = r0; r0 = r1; r1 = r2; r2 = sh;
+        sh = z0; z0 = z1; z1 = sh;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        P.stats->cycles = cycle;
+        P.stats->dv = dv;
+        P.stats->dp = dp;
+        P.stats->inner = inner;
+        P.stats->spmv = spmv;
+        P.stats->aborted = ok ? 0 : 1;
+        if (!(dv == dv) || !(dp == dp)) P.stats->nan_flag = 1;
+    }
+}
+
+// v_to_nodes (Solver.c:1319-1343)
+__global__ void k_v_to_nodes(LevelDev L, const double *__restrict__ d, double *__restrict__ V)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < L.nno; i += gridDim.x * blockDim.x)
+        for (int k = 0; k < 2; k++) {
+            int j = L.id[k * L.nno + i];
+            V[k * L.nno + i] = j >= 0 ? d[j] : L.v_bc_val[k * L.nno + i];
+        }
+}
+
+// compute_stress element part (Grain_size_evolution.c:154-209)
+__global__ void k_stress_elem(LevelDev L, const double *__restrict__ vis,
+                              const double *__restrict__ V, double *__restrict__ es, double *__restrict__ area)
+{
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < L.nel; e += gridDim.x * blockDim.x) {
+        int lev = L.elev[e];
+        double dxdxi = 0.5 * L.lvl_dx[lev], dzdeta = 0.5 * L.lvl_dz[lev];
+        double jac = dxdxi * dzdeta - 0.0 * 0.0;
+        double eta = 0.0, u11 = 0.0, u22 = 0.0, u12 = 0.0;
+        for (int a = 0; a < 4; a++) eta += 0.25 * vis[L.ienl[a * L.nel + e]];
+        for (int a = 0; a < 4; a++) {
+            int k = L.ienl[a * L.nel + e];
+            double ax = c_shape.dxic[a] * dzdeta - c_shape.detac[a] * 0.0;
+            double az = c_shape.detac[a] * dxdxi - c_shape.dxic[a] * 0.0;
+            u11 += ax * V[k];
+            u22 += az * V[L.nno + k];
+            u12 += az * V[k];
+        }
+        u11 /= jac;
+        u22 /= jac;
+        u12 /= jac;
+        double u21 = u12;
+        double sr = u11 * u11 + u22 * u22 + u12 * u21 * 2.0;
+        sr = sqrt(0.5 * sr);
+        es[e] = 2.0 * eta * sr;
+        int n0 = L.ienl[e], n2 = L.ienl[2 * L.nel + e];
+        area[e] = (L.X[n2] - L.X[n0]) * (L.X[L.nno + n2] - L.X[L.nno + n0]);
+    }
+}
+
+// p_to_nodes (Solver.c:1345-1366): area-weighted mean over the node's elements.
+__global__ void k_stress_node(LevelDev L, const double *__restrict__ es, const double *__restrict__ area,
+                              double *__restrict__ stress)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < L.nno; i += gridDim.x * blockDim.x) {
+        double s = 0.0, t = 0.0;
+        for (int k = 0; k < L.p2n_cnt[i]; k++) {
+            int e = L.p2n_el[k * L.nno + i];
+            s += area[e] * es[e];
+            t += area[e];
+        }
+        stress[i] = s / t;
+    }
+}
+
+// True residual norms after a solve: |F - K d - G p|^2, |G^T d|^2, |F|^2 as
+// partial sums in reduction slots 12..14 (k_collect3 finishes them).
+__global__ void k_residuals(LevelDev L, RedBuf rb, const double *__restrict__ d, const double *__restrict__ p)
+{
+    __shared__ dd lds[3][MCL_BLOCK / MCL_WAVE];
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsize = gridDim.x * blockDim.x;
+    dd acc[3] = {dd_zero(), dd_zero(), dd_zero()};
+    for (int i = gtid; i < L.neq; i += gsize) {
+        double r = L.F[i] - ell_row(L.Kcol, L.Kval, L.Kncol[i], L.neq, i, d) - g_row(L, i, p);
+        acc[0] = dd_add_prod(acc[0], r, r);
+        acc[2] = dd_add_prod(acc[2], L.F[i], L.F[i]);
+    }
+    for (int e = gtid; e < L.nel; e += gsize) {
+        double c = gt_elem(L, e, d);
+        acc[1] = dd_add_prod(acc[1], c, c);
+    }
+    reduce_publish<3>(rb, 12, acc, lds);
+}
+
+__global__ void k_collect3(RedBuf rb, int slot0, int nblocks, double *out)
+{
+    __shared__ dd lds[3][MCL_BLOCK / MCL_WAVE];
+    double o[3];
+    reduce_collect<3>(rb, slot0, nblocks, o, lds);
+    if (threadIdx.x == 0)
+        for (int r = 0; r < 3; r++) out[r] = o[r];
+}
+
+// ------------------------------------------------------------------ heat step
+
+// SUPG tau, calculate_artificial_diffusivity (Update_temperature.c:241-291),
+// and each element's advective time-step bound (advection_dt, :135-177, which
+// uses the sizes L of the last element of the diffusive loop for every element).
+__device__ __forceinline__ void elem_center_u(const LevelDev &L, int e, const double *__restrict__ V, double u[2])
+{
+    for (int d = 0; d < 2; d++) {
+        double s = 0.0;
+        for (int a = 0; a < 4; a++) s += 0.25 * V[d * L.nno + L.ienl[a * L.nel + e]];
+        u[d] = s;
+    }
+}
+
+// Heat row assembly (K_matrix_heat :293-423 with get_element_K_heat,
+// F_matrix_heat :425-508, C_matrix_heat :510-594): one thread per heat row,
+// element-ascending sums accumulated in place in the row's own slots.
+__device__ void heat_assemble_row(const LevelDev &L, const HeatParams &H, int P)
+{
+    const int n = L.heq_node[P];
+    const int nadj = L.p2n_cnt[n];
+    const int nT = L.neqT;
+    for (int s = 0; s < L.WT; s++) {
+        H.hK[s * nT + P] = 0.0;
+        H.hC[s * nT + P] = 0.0;
+    }
+    double F = 0.0;
+    for (int k = 0; k < nadj; k++) {
+        int e = L.p2n_el[k * L.nno + n], a = L.p2n_a[k * L.nno + n];
+        int lev = L.elev[e];
+        double dxdxi = 0.5 * L.lvl_dx[lev], dzdeta = 0.5 * L.lvl_dz[lev];
+        double jac = dxdxi * dzdeta - 0.0 * 0.0;
+        double u1 = 0.0, u2 = 0.0;
+        for (int c = 0; c < 4; c++) {
+            int kn = L.ienl[c * L.nel + e];
+            u1 += H.V[kn] * 0.25;
+            u2 += H.V[L.nno + kn] * 0.25;
+        }
+        double t0 = H.tau[e], t1 = H.tau[L.nel + e];
+        for (int b = 0; b < 4; b++) {
+            double kv = 0.0, cv = 0.0;
+            for (int g = 0; g < 4; g++) {
+                double ax = c_shape.dxi[a][g] * dzdeta - c_shape.deta[a][g] * 0.0;
+                double az = c_shape.deta[a][g] * dxdxi - c_shape.dxi[a][g] * 0.0;
+                double bx = c_shape.dxi[b][g] * dzdeta - c_shape.deta[b][g] * 0.0;
+                double bz = c_shape.deta[b][g] * dxdxi - c_shape.dxi[b][g] * 0.0;
+                double nh = t0 * ax + t1 * az;
+                double tt1 = u1 * c_shape.N[a][g] * bx + u2 * c_shape.N[a][g] * bz;
+                double tt2 = u1 * nh * bx + u2 * nh * bz;
+                double tt3 = ax * 1.0 * bx + ax * 0.0 * bz + az * 0.0 * bx + az * 1.0 * bz;
+                kv += tt1 + (tt2 + tt3) / jac;
+                cv += (c_shape.N[a][g] + nh) * c_shape.N[b][g];
+            }
+            cv *= jac;
+            int slot = L.Tslot[(k * 4 + b) * nT + P];
+            if (slot >= 0) {
+                H.hK[slot * nT + P] += kv;
+                H.hC[slot * nT + P] += cv;
+            }
+            int nb = L.ienl[b * L.nel + e];
+            if (L.T_bc[nb] == 'T') F -= kv * L.T_bc_val[nb];
+        }
+    }
+    for (int k = 0; k < nadj; k++) {
+        int e = L.p2n_el[k * L.nno + n], a = L.p2n_a[k * L.nno + n];
+        int lev = L.elev[e];
+        double dxdxi = 0.5 * L.lvl_dx[lev], dzdeta = 0.5 * L.lvl_dz[lev];
+        double jac = dxdxi * dzdeta - 0.0 * 0.0;
+        double t0 = H.tau[e], t1 = H.tau[L.nel + e];
+        int nat = 0;
+        for (int c = 0; c < 4; c++)
+            if (L.T_bc[L.ienl[c * L.nel + e]] == 'F') nat += c;
+        if (nat == 3) nat = 1;
+        else if (nat == 1 || nat == 5) nat = 2;
+        else if (nat == 0 || nat == 2) nat = 0;
+        double f = 0.0;
+        for (int g = 0; g < 4; g++) {
+            double ax = c_shape.dxi[a][g] * dzdeta - c_shape.deta[a][g] * 0.0;
+            double az = c_shape.deta[a][g] * dxdxi - c_shape.dxi[a][g] * 0.0;
+            double nh = t0 * ax + t1 * az;
+            f += (c_shape.N[a][g] + nh) * H.Q;
+        }
+        f *= jac;
+        if (nat != 0) f -= 0.5 * L.lvl_dz[lev] * L.T_bc_val[L.ienl[a * L.nel + e]];
+        F += f;
+    }
+    H.hF[P] = F;
+}
+
+// update_temperature (Update_temperature.c:3-133) as one persistent kernel.
+__global__ void __launch_bounds__(MCL_BLOCK) k_heat_fp64(HeatParams H)
+{
+    __shared__ dd lds[4][MCL_BLOCK / MCL_WAVE];
+    __shared__ double ldsm[MCL_BLOCK / MCL_WAVE];
+    const LevelDev &L = H.L;
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsize = gridDim.x * blockDim.x;
+    const int n = L.neqT, nel = L.nel;
+    unsigned gen = __hip_atomic_load(H.gs.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    bool ok = true;
+    long long spmv = 0;
+    // tau per element and the advective dt bound
+    double dtmin = 1e10;
+    for (int e = gtid; e < nel; e += gsize) {
+        int lev = L.elev[e];
+        double Lx = L.lvl_dx[lev], Lz = L.lvl_dz[lev];
+        double u[2];
+        elem_center_u(L, e, H.V, u);
+        double al[2], xi[2];
+        double Ld[2] = {Lx, Lz};
+        for (int d = 0; d < 2; d++) {
+            al[d] = 0.5 * u[d] * Ld[d] / 1.0;
+            xi[d] = fabs(al[d]) < H.accuracy ? 0.0 : cosh(al[d]) / sinh(al[d]) - 1.0 / al[d];
+        }
+        double us = (u[0] * u[0] + u[1] * u[1]);
+        double kh = (xi[0] * u[0] * Ld[0] + xi[1] * u[1] * Ld[1]) * 0.5;
+        H.tau[e] = kh * u[0] / us;
+        H.tau[nel + e] = kh * u[1] / us;
+        // advection_dt with the sizes of the last element (H.Llast)
+        double uu = fabs(u[0]) / H.Llast0 + fabs(u[1]) / H.Llast1;
+        double step = 0.5 / uu;
+        dtmin = step < dtmin ? step : dtmin;
+    }
+    {
+        double bm = block_min(dtmin, ldsm);
+        if (threadIdx.x == 0) H.rb.pmax[1 * MCL_MAX_WG + blockIdx.x] = bm;
+    }
+    ok = ok && grid_sync(H.gs, gen);
+    for (int P = gtid; P < n; P += gsize) heat_assemble_row(L, H, P);
+    double dt;
+    {
+        double m = 1e10;
+        for (int b = threadIdx.x; b < (int)gridDim.x; b += MCL_BLOCK) {
+            double v = H.rb.pmax[1 * MCL_MAX_WG + b];
+            m = v < m ? v : m;
+        }
+        dt = block_min(m, ldsm);
+        dt = dt < H.diff_dt ? dt : H.diff_dt;
+        dt = dt * 0.75;
+    }
+    ok = ok && grid_sync(H.gs, gen);
+    EllMat Kh{L.Tcol, L.Tncol, H.hK, nullptr, L.Tdiag_slot, n};
+    EllMat Ch{L.Tcol, L.Tncol, H.hC, nullptr, L.Tdiag_slot, n};
+    EllMat Ah{L.Tcol, L.Tncol, H.hA, nullptr, L.Tdiag_slot, n};
+    CgVecs cv{H.cg_r, H.cg_s, H.cg_d, H.cg_q};
+    int init_cycles = 0;
+    if (H.first_call) {
+        // initial_T_dot (:179-239)
+        for (int i = gtid; i < n; i += gsize) H.hd[i] = H.T[L.heq_node[i]];
+        ok = ok && grid_sync(H.gs, gen);
+        for (int i = gtid; i < n; i += gsize) {
+            H.B[i] = H.hF[i] - ell_row(L.Tcol, H.hK, L.Tncol[i], n, i, H.hd);
+            H.hdd[i] = 0.0;
+        }
+        ok = ok && grid_sync(H.gs, gen);
+        init_cycles = pcg_persistent(L, Ch, H.B, H.hdd, true, 1e-3, 400, H.cg_precondition, 1, cv, H.gs, gen,
+                                     H.rb, lds, ldsm, ok, spmv);
+    }
+    const double td = H.theta * dt;
+    for (int i = gtid; i < n; i += gsize) {
+        double ct = ell_row(L.Tcol, H.hC, L.Tncol[i], n, i, H.hd);
+        H.d0[i] = H.hd[i];
+        H.B[i] = H.hF[i] + ct / td;
+        for (int s = 0; s < L.Tncol[i]; s++) H.hA[s * n + i] = H.hC[s * n + i] / td + H.hK[s * n + i];
+    }
+    ok = ok && grid_sync(H.gs, gen);
+    int cycles = pcg_persistent(L, Ah, H.B, H.hd, false, 1e-3, 1000, H.cg_precondition, 1, cv, H.gs, gen, H.rb, lds,
+                                ldsm, ok, spmv);
+    for (int i = gtid; i < L.nno; i += gsize) {
+        int k = L.heat_id[i];
+        H.T[i] = k >= 0 ? H.hd[k] : L.T_bc_val[i];
+    }
+    for (int i = gtid; i < n; i += gsize)
+        H.hdd[i] = 1.0 / (H.theta * dt) * (H.hd[i] - H.d0[i]) - (1.0 - H.theta) / H.theta * H.hdd[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        H.stats->cycles = cycles;
+        H.stats->init_cycles = init_cycles;
+        H.stats->dt = dt;
+        H.stats->aborted = ok ? 0 : 1;
+        *H.model_time += dt;
+    }
+}

@@ -1,0 +1,106 @@
+// Device-side data layout of one quadtree level and the parameter blocks of
+// the persistent kernels.  Shared by the kernel translation units and the
+// host C-ABI implementation (mcl_api.cpp).
+#pragma once
+#include "mcl_device.h"
+
+// One quadtree level resident in HBM.  Arrays indexed [k][i] are stored
+// slot-major (k * n + i) so that consecutive threads (consecutive i) read
+// consecutive addresses.
+struct LevelDev {
+    int L, lev;
+    int nel, nno, neq, neqT, gnox, gnoz;
+    // level tables: element sizes per quadtree level and the element gradient
+    // vector g[level][p] of get_element_G (Build_matrix.c:346-381)
+    const double *lvl_dx, *lvl_dz;   // [MCL_MAX_LEVELS]
+    const double *gtab;              // [MCL_MAX_LEVELS][8]
+    // elements
+    const int *ienl;                 // [4][nel] local node index of corner a
+    const int *elev;                 // [nel] element quadtree level
+    const int *lm;                   // [8][nel] velocity eq of (dof d, corner a) at (d*4+a)
+    const int *gt_row;               // [8][nel] free velocity rows of the element, ascending, -1 padded
+    const signed char *gt_p;         // [8][nel] element dof index 2a+d of gt_row
+    // nodes
+    const int *node;                 // [nno] global node id
+    const int *p2n_cnt;              // [nno] number of elements at the node
+    const int *p2n_el;               // [4][nno] those elements, ascending
+    const int *p2n_a;                // [4][nno] corner index of the node in each
+    const int *id;                   // [2][nno]
+    const int *heat_id;              // [nno]
+    const signed char *v_bc;         // [2][nno]
+    const double *v_bc_val;          // [2][nno]
+    const signed char *T_bc;         // [nno]
+    const double *T_bc_val;          // [nno]
+    const double *X;                 // [2][nno]
+    // velocity rows (ELL, sorted columns)
+    const int *veq_node;             // [neq] local node of the equation
+    const int *veq_dof;              // [neq]
+    int WK;
+    const int *Kcol;                 // [WK][neq]
+    const int *Kncol;                // [neq]
+    const signed char *Kslot;        // [32][neq]: slot of (adjacent element k, element dof q)
+    double *Kval;                    // [WK][neq]
+    double *Kdiag, *F;               // [neq]
+    // heat rows
+    const int *heq_node;             // [neqT] local node of the heat equation
+    int WT;
+    const int *Tcol;                 // [WT][neqT]
+    const int *Tncol;                // [neqT]
+    const signed char *Tslot;        // [16][neqT]
+    const int *Tdiag_slot;           // [neqT]
+    // hanging nodes
+    int nh, h_serial;
+    const int *h_ni, *h_ci;          // [nh], [2][nh]
+};
+
+struct DevStats {
+    int cycles, init_cycles, aborted, nan_flag;
+    double dv, dp, dt;
+    long long inner, spmv;
+};
+
+struct UzawaParams {
+    LevelDev L;
+    GridSync gs;
+    RedBuf rb;
+    DevStats *stats;
+    double *d, *p, *u1, *Ahv, *Ahp, *BAB;
+    double *r0, *r1, *r2, *z0, *z1, *s1, *s2;
+    double *cg_r, *cg_s, *cg_d, *cg_q;
+    double accuracy, citcom_accu;
+    int max_citcom_cycle, max_cg_cycle, cg_precondition, check_v_only, schur;
+};
+
+struct HeatParams {
+    LevelDev L;
+    GridSync gs;
+    RedBuf rb;
+    DevStats *stats;
+    const double *V;
+    double *T, *tau, *hK, *hC, *hA, *hF, *hd, *hdd, *d0, *B;
+    double *cg_r, *cg_s, *cg_d, *cg_q;
+    double *model_time;
+    double accuracy, theta, Q, Llast0, Llast1, diff_dt;
+    int first_call, cg_precondition;
+};
+
+// Reduction slots in RedBuf (each slot holds MCL_MAX_WG partials).
+#define MCL_RED_SLOTS 16
+
+extern "C" hipError_t mcl_upload_shape(const ShapeTables *s);
+
+__global__ void k_viscosity(int nno, const double *T, const double *X, double A, double refT, double lm_jump,
+                            double *vis);
+__global__ void k_buoy_rows(int gnoz, int gnox, int nno, const int *ptr, const int *ent_e,
+                            const signed char *ent_xi, const signed char *ent_eta, const int *ienl, int nel,
+                            const double *T, double Ra, double *avg);
+__global__ void k_buoy_apply(int nno, int gnoz, const int *node, const double *T, double Ra, const double *avg,
+                             double *buoy);
+__global__ void k_assemble_K(LevelDev L, const double *vis, const double *buoy);
+__global__ void k_uzawa_fp64(UzawaParams P);
+__global__ void k_v_to_nodes(LevelDev L, const double *d, double *V);
+__global__ void k_stress_elem(LevelDev L, const double *vis, const double *V, double *es, double *area);
+__global__ void k_stress_node(LevelDev L, const double *es, const double *area, double *stress);
+__global__ void k_residuals(LevelDev L, RedBuf rb, const double *d, const double *p);
+__global__ void k_collect3(RedBuf rb, int slot0, int nblocks, double *out);
+__global__ void k_heat_fp64(HeatParams H);

@@ -1,0 +1,27 @@
+"""Golden cases shared by the CPU and GPU tests."""
+import numpy as np
+
+from tests.oracle_lib import fixture_keys, load_fixture
+
+# fixtures with a full first step (matrices, vectors, counts)
+STEP_CASES = ["c1_64", "c2s_32", "plain_cg_16", "dev12_32", "dev8_16", "dev14_16", "c2s_64"]
+MESH_CASES = STEP_CASES + ["mg_16"]
+
+
+def case_input(name):
+    """(fixture, Input) for a golden case; seeded-T cases read T from the fixture."""
+    from mantle_convection_lab_amd import Input
+    fx = load_fixture(name)
+    keys = fixture_keys(fx)
+    keys.pop("old_T_timestep", None)
+    if keys.get("T_ini") == "-1":
+        keys["T_ini"] = "0"   # the fixture supplies T; the mesh does not depend on it
+    keys["write_output"] = "0"
+    return fx, Input.from_keys(keys)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)

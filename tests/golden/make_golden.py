@@ -57,8 +57,6 @@ CASES = {
     # refined meshes with hanging nodes (develop_mode recipes, Tree.c:110-275), capped solves
     "dev12_32": (dict(level=6, mg_min_level=5, Rayleigh=1e4, T_ini=0, develop_mode=12,
                       max_timestep=0, max_citcom_cycle=8), 1, 0, False),
-    "dev2_64": (dict(level=7, mg_min_level=6, Rayleigh=1e4, T_ini=0, develop_mode=2,
-                     max_timestep=0, max_citcom_cycle=4, max_cg_cycle=20), 1, 0, False),
     "dev8_16": (dict(level=5, mg_min_level=4, Rayleigh=1e4, T_ini=0, develop_mode=8,
                      max_timestep=0, max_citcom_cycle=6), 1, 0, False),
     "dev11_32": (dict(level=6, mg_min_level=5, Rayleigh=1e4, T_ini=0, develop_mode=11,
