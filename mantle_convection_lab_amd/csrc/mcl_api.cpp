@@ -41,6 +41,13 @@ struct Level {
     // device fields
     double *T = nullptr, *vis = nullptr, *buoy = nullptr, *V = nullptr, *d = nullptr, *p = nullptr;
     double *stress = nullptr, *es = nullptr, *area = nullptr, *avg = nullptr;
+    // multigrid (mixed precision): prolongation maps of this level (fine side)
+    std::vector<int> interp_n, interp_eq;
+    int *c2f = nullptr;                       // coarse node -> finest local node
+    float *val32 = nullptr, *dinv32 = nullptr, *mx = nullptr, *mb = nullptr, *mr = nullptr, *mt = nullptr;
+    int *pn = nullptr, *pcol = nullptr, *rcol = nullptr;
+    float *rval = nullptr;
+    int rw = 0;
     // buoyancy row lists
     int *brow_ptr = nullptr, *brow_e = nullptr;
     signed char *brow_xi = nullptr, *brow_eta = nullptr;
@@ -69,6 +76,13 @@ struct mcl_ctx {
     size_t work_doubles = 0;
     double *model_time = nullptr;    // device scalar
     double *resid = nullptr;         // device [3]
+    // mixed precision: multigrid level table, smoother bounds, coarse inverse, FP32 CG vectors
+    MgLevelDev *mg_dev = nullptr;
+    unsigned *lam_bits = nullptr;
+    double *cinv = nullptr;
+    float *cg32 = nullptr;
+    double *rr = nullptr;
+    bool have_mg = false;
     // heat work (finest level)
     double *tau = nullptr, *hK = nullptr, *hC = nullptr, *hA = nullptr, *hF = nullptr;
     double *hd = nullptr, *hdd = nullptr, *hd0 = nullptr, *hB = nullptr;
@@ -173,7 +187,10 @@ extern "C" int mcl_create(int device, mcl_ctx **out)
     int per_cu_u = 0, per_cu_h = 0;
     hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_u, (const void *)k_uzawa_fp64, MCL_BLOCK, 0);
     hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_h, (const void *)k_heat_fp64, MCL_BLOCK, 0);
-    c->max_coop_blocks = std::min(std::max(1, std::min(per_cu_u, per_cu_h)) * c->num_cu, MCL_MAX_WG);
+    int per_cu_m = 0;
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_m, (const void *)k_uzawa_mixed, MCL_BLOCK, 0);
+    int per_cu = std::min(std::min(per_cu_u, per_cu_h), per_cu_m);
+    c->max_coop_blocks = std::min(std::max(1, per_cu) * c->num_cu, MCL_MAX_WG);
     // grid barrier and reduction buffers
     unsigned *sync = nullptr;
     if (dev_alloc(c, c->owned, &sync, 4)) { delete c; return MCL_E_DEVICE; }
@@ -373,6 +390,10 @@ static int build_level(mcl_ctx *c, const mcl_mesh_view &v, Level &Lv)
     Lv.WT = WT;
     Lv.Tncol_h = Tncol;
     Lv.Tcol_h = Tcol;
+    if (v.interp_n && v.interp_eq) {
+        Lv.interp_n.assign(v.interp_n, v.interp_n + neq);
+        Lv.interp_eq.assign(v.interp_eq, v.interp_eq + 4 * (size_t)neq);
+    }
     // buoyancy row-average contribution lists, in the reference's order
     // (get_node_horiz_avg, Parallel.c:354-437)
     std::vector<std::vector<int>> be(v.gnoz);
@@ -523,6 +544,136 @@ static int build_level(mcl_ctx *c, const mcl_mesh_view &v, Level &Lv)
     return rc ? (c->err.empty() ? fail(c, MCL_E_DEVICE, "device allocation failed") : MCL_E_DEVICE) : 0;
 }
 
+// Multigrid data for the mixed-precision inner solver: per level an FP32 copy
+// of K, prolongation from the level below (the reference's interp maps), the
+// restriction R = P^T stored on the coarse level, level vectors, and the map of
+// coarse nodes to finest nodes used to inject the temperature.
+static int block_rows_threshold()
+{
+    const char *e = getenv("MCL_MG_BLOCK_ROWS");
+    return e ? atoi(e) : 1024;
+}
+
+static int build_multigrid(mcl_ctx *c)
+{
+    c->have_mg = false;
+    const int nlev = c->nlev;
+    Level &Fn = c->levels.back();
+    std::vector<MgLevelDev> mg(nlev);
+    for (int k = 0; k < nlev; k++) {
+        Level &Lv = c->levels[k];
+        std::vector<void *> &o = Lv.owned;
+        int n = std::max(Lv.neq, 1);
+        int rc = 0;
+        rc |= dev_alloc(c, o, &Lv.val32, (size_t)std::max(Lv.WK, 1) * n);
+        rc |= dev_alloc(c, o, &Lv.dinv32, n);
+        rc |= dev_alloc(c, o, &Lv.mx, n);
+        rc |= dev_alloc(c, o, &Lv.mb, n);
+        rc |= dev_alloc(c, o, &Lv.mr, n);
+        rc |= dev_alloc(c, o, &Lv.mt, n);
+        if (rc) return MCL_E_DEVICE;
+        MgLevelDev &M = mg[k];
+        M.n = Lv.neq;
+        M.WK = Lv.WK;
+        M.block = (k == 0 || Lv.neq <= block_rows_threshold()) ? 1 : 0;
+        M.col = Lv.dev.Kcol;
+        M.ncol = Lv.dev.Kncol;
+        M.val = Lv.val32;
+        M.dinv = Lv.dinv32;
+        M.val64 = Lv.dev.Kval;
+        M.diag64 = Lv.dev.Kdiag;
+        M.x = Lv.mx;
+        M.b = Lv.mb;
+        M.r = Lv.mr;
+        M.t = Lv.mt;
+        // the coarse node -> finest node map
+        if (k < nlev - 1) {
+            std::vector<int> c2f(Lv.nno);
+            for (int i = 0; i < Lv.nno; i++) {
+                c2f[i] = local_of(Fn.node, Lv.node[i]);
+                if (c2f[i] < 0) return fail(c, MCL_E_ARG, "level %d node %d not on the finest level", Lv.lev, i);
+            }
+            const int *cp;
+            if (dev_upload(c, o, &cp, c2f)) return MCL_E_DEVICE;
+            Lv.c2f = (int *)cp;
+        }
+    }
+    bool maps = nlev > 1;
+    for (int k = 1; k < nlev; k++)
+        if (c->levels[k].interp_n.empty()) maps = false;
+    if (maps) {
+        for (int k = 1; k < nlev; k++) {
+            Level &F = c->levels[k], &C = c->levels[k - 1];
+            const int nf = F.neq, nc = C.neq;
+            // prolongation
+            std::vector<int> pcol(4 * (size_t)std::max(nf, 1), -1);
+            for (int a = 0; a < 4; a++)
+                for (int u = 0; u < nf; u++)
+                    if (F.interp_n[u] > a) pcol[a * (size_t)nf + u] = F.interp_eq[a * (size_t)nf + u];
+            const int *cp;
+            if (dev_upload(c, F.owned, &cp, F.interp_n)) return MCL_E_DEVICE;
+            F.pn = (int *)cp;
+            if (dev_upload(c, F.owned, &cp, pcol)) return MCL_E_DEVICE;
+            F.pcol = (int *)cp;
+            mg[k].pn = F.pn;
+            mg[k].pcol = F.pcol;
+            // restriction = transpose, rows of the coarse level
+            std::vector<std::vector<std::pair<int, float>>> rows(std::max(nc, 1));
+            for (int u = 0; u < nf; u++) {
+                int m = F.interp_n[u];
+                for (int a = 0; a < m; a++) {
+                    int l = F.interp_eq[a * (size_t)nf + u];
+                    if (l >= 0) rows[l].push_back({u, 1.0f / (float)m});
+                }
+            }
+            int rw = 1;
+            for (auto &r : rows) rw = std::max<int>(rw, (int)r.size());
+            std::vector<int> rcol((size_t)rw * std::max(nc, 1), -1);
+            std::vector<float> rval((size_t)rw * std::max(nc, 1), 0.f);
+            for (int l = 0; l < nc; l++)
+                for (size_t t = 0; t < rows[l].size(); t++) {
+                    rcol[t * nc + l] = rows[l][t].first;
+                    rval[t * nc + l] = rows[l][t].second;
+                }
+            const float *fp;
+            if (dev_upload(c, C.owned, &cp, rcol)) return MCL_E_DEVICE;
+            C.rcol = (int *)cp;
+            if (dev_upload(c, C.owned, &fp, rval)) return MCL_E_DEVICE;
+            C.rval = (float *)fp;
+            C.rw = rw;
+            mg[k - 1].rw = rw;
+            mg[k - 1].rcol = C.rcol;
+            mg[k - 1].rval = C.rval;
+        }
+        if (c->levels[0].neq > 64) maps = false;  // the coarse solve holds a dense inverse of at most 64 rows
+    }
+    for (void *p : {(void *)c->mg_dev, (void *)c->lam_bits, (void *)c->cinv, (void *)c->cg32, (void *)c->rr}) {
+        if (p) {
+            (void)hipFree(p);
+            c->owned.erase(std::remove(c->owned.begin(), c->owned.end(), p), c->owned.end());
+        }
+    }
+    c->mg_dev = nullptr;
+    int rc = 0;
+    if (maps) {
+        rc |= dev_alloc(c, c->owned, &c->mg_dev, nlev);
+        if (!rc) rc |= hipMemcpy(c->mg_dev, mg.data(), nlev * sizeof(MgLevelDev), hipMemcpyHostToDevice) != hipSuccess;
+    } else {
+        // single-level Jacobi-preconditioned FP32 CG: only the finest entry is used
+        rc |= dev_alloc(c, c->owned, &c->mg_dev, 1);
+        MgLevelDev top = mg[nlev - 1];
+        top.block = 0;
+        if (!rc) rc |= hipMemcpy(c->mg_dev, &top, sizeof(MgLevelDev), hipMemcpyHostToDevice) != hipSuccess;
+    }
+    rc |= dev_alloc(c, c->owned, &c->lam_bits, MCL_MAX_LEVELS);
+    rc |= dev_alloc(c, c->owned, &c->cinv, 64 * 64);
+    rc |= dev_alloc(c, c->owned, &c->cg32, 5 * (size_t)std::max(Fn.neq, 1));
+    rc |= dev_alloc(c, c->owned, &c->rr, std::max(Fn.neq, 1));
+    if (rc) return fail(c, MCL_E_DEVICE, "multigrid allocation failed");
+    c->have_mg = maps;
+    return 0;
+}
+
 extern "C" int mcl_mesh_upload(mcl_ctx *c, const mcl_mesh_view *levels, int nlev)
 {
     if (!c) return MCL_E_ARG;
@@ -542,6 +693,10 @@ extern "C" int mcl_mesh_upload(mcl_ctx *c, const mcl_mesh_view *levels, int nlev
         }
     }
     c->nlev = nlev;
+    {
+        int rc = build_multigrid(c);
+        if (rc) return rc;
+    }
     // finest-level solver work space
     Level &F = c->levels[nlev - 1];
     for (void *p : {(void *)c->work, (void *)c->tau, (void *)c->hK, (void *)c->hC, (void *)c->hA, (void *)c->hF,
@@ -576,8 +731,8 @@ extern "C" int mcl_set_control(mcl_ctx *c, const mcl_control *ctl)
     if (ctl->solver < 0 || ctl->solver > 3) return fail(c, MCL_E_ARG, "solver %d", ctl->solver);
     if (ctl->solver == MCL_SOLVER_JC || ctl->solver == MCL_SOLVER_GS || ctl->solver == MCL_SOLVER_MG)
         return fail(c, MCL_E_UNSUPPORTED, "inner solver %d not available on the device path yet", ctl->solver);
-    if (ctl->precision != MCL_PREC_FP64)
-        return fail(c, MCL_E_UNSUPPORTED, "precision %d not available yet", ctl->precision);
+    if (ctl->precision != MCL_PREC_FP64 && ctl->precision != MCL_PREC_MIXED)
+        return fail(c, MCL_E_ARG, "precision %d", ctl->precision);
     if (ctl->theta == 0.0) return fail(c, MCL_E_ARG, "theta = 0 divides by zero in the heat step");
     c->ctl = *ctl;
     c->have_ctl = true;
@@ -753,11 +908,29 @@ extern "C" int mcl_viscosity(mcl_ctx *c)
     return 0;
 }
 
+static int inject_temperature(mcl_ctx *c)
+{
+    Level &F = c->levels.back();
+    for (int k = 0; k + 1 < c->nlev; k++) {
+        Level &Lv = c->levels[k];
+        hipLaunchKernelGGL(k_inject, dim3(grid_for(c, Lv.nno)), dim3(MCL_BLOCK), 0, c->stream, Lv.nno,
+                           (const int *)Lv.c2f, (const double *)F.T, Lv.T);
+        HIPCHK(c, hipGetLastError());
+    }
+    return 0;
+}
+
 extern "C" int mcl_buoyancy_force(mcl_ctx *c)
 {
     int rc = ready(c);
     if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->device));
+    // The mixed path rediscretises the coarse levels from the current
+    // temperature (the reference keeps the initial one there).
+    if (c->ctl.precision == MCL_PREC_MIXED && c->nlev > 1) {
+        rc = inject_temperature(c);
+        if (rc) return rc;
+    }
     for (auto &L : c->levels) {
         hipLaunchKernelGGL(k_buoy_rows, dim3((L.gnoz + MCL_BLOCK - 1) / MCL_BLOCK), dim3(MCL_BLOCK), 0, c->stream,
                            L.gnoz, L.gnox, L.nno, L.brow_ptr, L.brow_e, L.brow_xi, L.brow_eta, L.dev.ienl, L.nel,
@@ -867,9 +1040,55 @@ extern "C" int mcl_solve_velocity_pressure(mcl_ctx *c, mcl_stokes_stats *out)
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
     HIPCHK(c, hipEventRecord(e0, c->stream));
-    void *args[] = {&P};
-    HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_uzawa_fp64, dim3(P.gs.nblocks), dim3(MCL_BLOCK), args, 0,
-                                         c->stream));
+    if (k.precision == MCL_PREC_MIXED) {
+        // FP32 copies of K and smoother bounds per multigrid level, dense
+        // inverse of the coarsest K, then the persistent mixed Uzawa kernel.
+        const int nm = c->have_mg ? c->nlev : 1;
+        HIPCHK(c, hipMemsetAsync(c->lam_bits, 0, MCL_MAX_LEVELS * sizeof(unsigned), c->stream));
+        for (int m = 0; m < nm; m++) {
+            Level &Lv = c->have_mg ? c->levels[m] : c->levels.back();
+            if (Lv.neq == 0) continue;
+            hipLaunchKernelGGL(k_to_fp32, dim3(grid_for(c, Lv.neq)), dim3(MCL_BLOCK), 0, c->stream, Lv.neq, Lv.WK,
+                               (const double *)Lv.dev.Kval, (const double *)Lv.dev.Kdiag, Lv.val32, Lv.dinv32);
+            hipLaunchKernelGGL(k_gershgorin, dim3(grid_for(c, Lv.neq)), dim3(MCL_BLOCK), 0, c->stream, Lv.neq,
+                               Lv.WK, (const double *)Lv.dev.Kval, (const double *)Lv.dev.Kdiag, c->lam_bits + m);
+            HIPCHK(c, hipGetLastError());
+        }
+        if (c->have_mg) {
+            Level &C0 = c->levels[0];
+            hipLaunchKernelGGL(k_coarse_inverse, dim3(1), dim3(MCL_BLOCK), 0, c->stream, C0.neq, C0.WK,
+                               C0.dev.Kcol, C0.dev.Kncol, (const double *)C0.dev.Kval, c->cinv);
+            HIPCHK(c, hipGetLastError());
+        }
+        MixedParams M{};
+        M.U = P;
+        M.mg = c->mg_dev;
+        M.lam = (const float *)c->lam_bits;
+        M.nmg = nm;
+        M.cinv = c->cinv;
+        M.nc = c->levels[0].neq;
+        size_t nv32 = std::max(L.neq, 1);
+        M.cg_r = c->cg32;
+        M.cg_z = c->cg32 + nv32;
+        M.cg_d = c->cg32 + 2 * nv32;
+        M.cg_q = c->cg32 + 3 * nv32;
+        M.cg_e = c->cg32 + 4 * nv32;
+        M.rr = c->rr;
+        M.inner_rtol = k.inner_rtol > 0 ? k.inner_rtol : 1e-8;
+        M.cg_rtol = 1e-4f;
+        M.max_refine = 10;
+        M.max_cg32 = 500;
+        M.pre = 2;
+        M.post = 2;
+        M.omega = 4.0f / 3.0f;
+        void *margs[] = {&M};
+        HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_uzawa_mixed, dim3(P.gs.nblocks), dim3(MCL_BLOCK), margs,
+                                             0, c->stream));
+    } else {
+        void *args[] = {&P};
+        HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_uzawa_fp64, dim3(P.gs.nblocks), dim3(MCL_BLOCK), args,
+                                             0, c->stream));
+    }
     HIPCHK(c, hipEventRecord(e1, c->stream));
     // nodal velocity and stress (Solver.c:8-9)
     for (auto &Lv : c->levels) {

@@ -729,8 +729,9 @@ int mclh_build_mesh(const mclh_input *in, mclh_mesh **out, char *err, int errlen
         free(codes);
         free(ncodes);
     }
-    if (!strcmp(in->solver, "mg"))
-        for (k = 1; k < m->nlev; k++) build_mg_maps(&m->lev[k - 1], &m->lev[k]);
+    /* transfer maps for every multi-level mesh (the reference builds them for
+     * solver = "mg"; the mixed-precision path uses them as its multigrid) */
+    for (k = 1; k < m->nlev; k++) build_mg_maps(&m->lev[k - 1], &m->lev[k]);
     *out = m;
     return 0;
 }

@@ -84,6 +84,44 @@ struct HeatParams {
     int first_call, cg_precondition;
 };
 
+// One multigrid level of the FP32 inner solver (mixed precision).  Level 0 is
+// the coarsest; the finest level shares its rows with the Stokes velocity
+// equations.  Levels with block != 0 are small enough to be processed by one
+// workgroup (LDS-speed barriers); the others are processed by the whole grid.
+struct MgLevelDev {
+    int n, WK, block;
+    const int *col;          // [WK][n] (the level's FP64 ELL pattern)
+    const int *ncol;         // [n]
+    float *val;              // [WK][n] FP32 copy of the level's K
+    float *dinv;             // [n] 1 / diag(K)
+    const double *val64;     // [WK][n] the FP64 K of the level
+    const double *diag64;    // [n]
+    // prolongation from level-1: fine row u <- mean of pn[u] coarse rows pcol[k][u] (-1 = fixed)
+    const int *pn;           // [n]
+    const int *pcol;         // [4][n]
+    // restriction to level-1 (= P^T), stored on this level's coarse partner:
+    // coarse row c <- sum_k rval[k][c] * r_fine[rcol[k][c]]
+    int rw;
+    const int *rcol;         // [rw][n_coarse]
+    const float *rval;       // [rw][n_coarse]
+    float *x, *b, *r, *t;    // level vectors
+};
+
+struct MixedParams {
+    UzawaParams U;
+    const MgLevelDev *mg;    // [nmg] (device memory)
+    const float *lam;        // [nmg] Gershgorin bounds of D^-1 K per level
+    int nmg;
+    const double *cinv;      // dense inverse of the coarsest K [nc][nc]
+    int nc;
+    float *cg_r, *cg_z, *cg_d, *cg_q, *cg_e;   // FP32 inner CG vectors (finest rows)
+    double *rr;              // FP64 refinement residual
+    double inner_rtol;       // FP64 relative tolerance of each inner solve
+    float cg_rtol;           // relative tolerance of each FP32 CG sweep
+    int max_refine, max_cg32, pre, post;
+    float omega;             // damped-Jacobi weight of the smoother
+};
+
 // Reduction slots in RedBuf (each slot holds MCL_MAX_WG partials).
 #define MCL_RED_SLOTS 16
 
@@ -104,3 +142,8 @@ __global__ void k_stress_node(LevelDev L, const double *es, const double *area, 
 __global__ void k_residuals(LevelDev L, RedBuf rb, const double *d, const double *p);
 __global__ void k_collect3(RedBuf rb, int slot0, int nblocks, double *out);
 __global__ void k_heat_fp64(HeatParams H);
+__global__ void k_uzawa_mixed(MixedParams P);
+__global__ void k_to_fp32(int n, int W, const double *val, const double *diag, float *val32, float *dinv);
+__global__ void k_coarse_inverse(int n, int W, const int *col, const int *ncol, const double *val, double *inv);
+__global__ void k_inject(int nc, const int *c2f, const double *fine, double *coarse);
+__global__ void k_gershgorin(int n, int W, const double *val, const double *diag, unsigned *bound_bits);
