@@ -205,6 +205,40 @@ int mcl_update_temperature(mcl_ctx *ctx, mcl_heat_stats *stats);        /* Updat
 /* Synchronise the context's stream. */
 int mcl_synchronize(mcl_ctx *ctx);
 
+/* ---------------------------------------------------------------------------
+ * Learned initial guess (no counterpart in the reference: new in this library).
+ * A U-Net maps temperature and log10 viscosity at the element centres of a
+ * uniform H x W element mesh to u_x, u_z, p / Ra.  It is trained on the device
+ * on snapshots of the solver's own fields; mcl_stokes_guess proposes (d, p)
+ * for the next solve and keeps the current ones when the network's momentum
+ * residual is larger.
+ */
+typedef struct mcl_net mcl_net;
+
+int mcl_net_create(int H, int W, int batch, unsigned long long seed, mcl_net **out);
+void mcl_net_destroy(mcl_net *net);
+const char *mcl_net_last_error(const mcl_net *net);
+long long mcl_net_num_params(const mcl_net *net);
+/* flat FP32 parameters: per convolution w[Cout][3][3][Cin], b[Cout]; head w[3][32], b[3] */
+int mcl_net_get_params(mcl_net *net, float *host);
+int mcl_net_set_params(mcl_net *net, const float *host);
+int mcl_net_get_grads(mcl_net *net, float *host);
+/* output scale (RMS of the training targets, set by the first mcl_net_train) */
+double mcl_net_get_scale(const mcl_net *net);
+int mcl_net_set_scale(mcl_net *net, double scale);
+/* host arrays: x [B][H][W][2], y and target [B][H][W][3] */
+int mcl_net_forward(mcl_net *net, const float *x_host, float *y_host);
+/* one Adam step on the mean squared error (lr = 0: gradients only); loss is the batch MSE */
+int mcl_net_train_step(mcl_net *net, const float *x_host, const float *t_host, float lr, float *loss);
+/* capture the context's current T, viscosity, velocity and pressure as a snapshot */
+int mcl_net_capture(mcl_ctx *ctx, mcl_net *net);
+long long mcl_net_dataset_size(const mcl_net *net);
+/* epochs of shuffled mini-batches over the captured snapshots; losses[epoch] = mean batch loss */
+int mcl_net_train(mcl_net *net, int epochs, float lr, unsigned long long seed, float *losses);
+/* propose the network's (d, p) for the next solve of ctx; *accepted = 1 if it replaced the
+ * current guess (smaller |F - K d - G p|); residual norms of both are returned if non-NULL */
+int mcl_stokes_guess(mcl_ctx *ctx, mcl_net *net, int *accepted, double *res_net, double *res_prev);
+
 #ifdef __cplusplus
 }
 #endif

@@ -158,3 +158,29 @@ def host_lib():
         L.mclh_diagnostics.argtypes = [C.POINTER(HostInput), C.POINTER(HostLevel), PD, PD, PD, PD]
         _host = L
     return _host
+
+
+def declare_net(L):
+    """Declarations of the learned-guess entry points of mcl.h."""
+    vp = C.c_void_p
+    PF = C.POINTER(C.c_float)
+    L.mcl_net_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(vp)]
+    L.mcl_net_destroy.argtypes = [vp]
+    L.mcl_net_destroy.restype = None
+    L.mcl_net_last_error.argtypes = [vp]
+    L.mcl_net_last_error.restype = C.c_char_p
+    L.mcl_net_num_params.argtypes = [vp]
+    L.mcl_net_num_params.restype = C.c_longlong
+    for f in ("mcl_net_get_params", "mcl_net_set_params", "mcl_net_get_grads"):
+        getattr(L, f).argtypes = [vp, PF]
+    L.mcl_net_forward.argtypes = [vp, PF, PF]
+    L.mcl_net_train_step.argtypes = [vp, PF, PF, C.c_float, PF]
+    L.mcl_net_capture.argtypes = [vp, vp]
+    L.mcl_net_dataset_size.argtypes = [vp]
+    L.mcl_net_dataset_size.restype = C.c_longlong
+    L.mcl_net_train.argtypes = [vp, C.c_int, C.c_float, C.c_ulonglong, PF]
+    L.mcl_stokes_guess.argtypes = [vp, vp, C.POINTER(C.c_int), PD, PD]
+    L.mcl_net_get_scale.argtypes = [vp]
+    L.mcl_net_get_scale.restype = C.c_double
+    L.mcl_net_set_scale.argtypes = [vp, C.c_double]
+    return L

@@ -24,7 +24,8 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", 
 
 # (source, extra flags): -ffp-contract=off keeps the FP64 parity kernels free of
 # FMA contraction, so they round like the reference's x86-64 build.
-HIP_UNITS = [("mcl_fp64.hip", ["-ffp-contract=off"]), ("mcl_mixed.hip", []), ("mcl_api.cpp", ["-x", "hip"])]
+HIP_UNITS = [("mcl_fp64.hip", ["-ffp-contract=off"]), ("mcl_mixed.hip", []), ("mcl_unet.hip", []),
+             ("mcl_net.cpp", ["-x", "hip"]), ("mcl_api.cpp", ["-x", "hip"])]
 
 
 def run(cmd, quiet=False):

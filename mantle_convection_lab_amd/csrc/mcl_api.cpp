@@ -18,8 +18,9 @@
 #include <string>
 #include <vector>
 
-#include "../../include/mcl.h"
+#include "mcl.h"
 #include "mcl_kernels.h"
+#include "mcl_net.h"
 
 namespace {
 
@@ -80,7 +81,6 @@ struct mcl_ctx {
     MgLevelDev *mg_dev = nullptr;
     unsigned *lam_bits = nullptr;
     double *cinv = nullptr;
-    float *cg32 = nullptr;
     double *rr = nullptr;
     bool have_mg = false;
     // heat work (finest level)
@@ -193,10 +193,10 @@ extern "C" int mcl_create(int device, mcl_ctx **out)
     c->max_coop_blocks = std::min(std::max(1, per_cu) * c->num_cu, MCL_MAX_WG);
     // grid barrier and reduction buffers
     unsigned *sync = nullptr;
-    if (dev_alloc(c, c->owned, &sync, 4)) { delete c; return MCL_E_DEVICE; }
-    c->gs.count = sync;
-    c->gs.gen = sync + 1;
-    c->gs.abort = sync + 2;
+    if (dev_alloc(c, c->owned, &sync, MCL_SYNC_WORDS)) { delete c; return MCL_E_DEVICE; }
+    c->gs.gen = sync;
+    c->gs.abort = sync + 32;
+    c->gs.arrive = sync + 64;
     if (dev_alloc(c, c->owned, &c->rb.part, (size_t)MCL_RED_SLOTS * MCL_MAX_WG) ||
         dev_alloc(c, c->owned, &c->rb.pmax, (size_t)MCL_RED_SLOTS * MCL_MAX_WG) ||
         dev_alloc(c, c->owned, &c->stats, 1) || dev_alloc(c, c->owned, &c->model_time, 1) ||
@@ -647,7 +647,7 @@ static int build_multigrid(mcl_ctx *c)
         }
         if (c->levels[0].neq > 64) maps = false;  // the coarse solve holds a dense inverse of at most 64 rows
     }
-    for (void *p : {(void *)c->mg_dev, (void *)c->lam_bits, (void *)c->cinv, (void *)c->cg32, (void *)c->rr}) {
+    for (void *p : {(void *)c->mg_dev, (void *)c->lam_bits, (void *)c->cinv, (void *)c->rr}) {
         if (p) {
             (void)hipFree(p);
             c->owned.erase(std::remove(c->owned.begin(), c->owned.end(), p), c->owned.end());
@@ -659,7 +659,7 @@ static int build_multigrid(mcl_ctx *c)
         rc |= dev_alloc(c, c->owned, &c->mg_dev, nlev);
         if (!rc) rc |= hipMemcpy(c->mg_dev, mg.data(), nlev * sizeof(MgLevelDev), hipMemcpyHostToDevice) != hipSuccess;
     } else {
-        // single-level Jacobi-preconditioned FP32 CG: only the finest entry is used
+        // single level: Jacobi preconditioner from the finest entry's FP32 1/diag
         rc |= dev_alloc(c, c->owned, &c->mg_dev, 1);
         MgLevelDev top = mg[nlev - 1];
         top.block = 0;
@@ -667,7 +667,6 @@ static int build_multigrid(mcl_ctx *c)
     }
     rc |= dev_alloc(c, c->owned, &c->lam_bits, MCL_MAX_LEVELS);
     rc |= dev_alloc(c, c->owned, &c->cinv, 64 * 64);
-    rc |= dev_alloc(c, c->owned, &c->cg32, 5 * (size_t)std::max(Fn.neq, 1));
     rc |= dev_alloc(c, c->owned, &c->rr, std::max(Fn.neq, 1));
     if (rc) return fail(c, MCL_E_DEVICE, "multigrid allocation failed");
     c->have_mg = maps;
@@ -957,17 +956,22 @@ extern "C" int mcl_velocity_pressure_KGF_matrix(mcl_ctx *c)
     return 0;
 }
 
+// Workgroups of a persistent solve: one per 256 rows, at most what fits
+// co-resident, and at most MCL_COOP_BLOCKS if set (a grid barrier costs more
+// with more workgroups).
 static unsigned coop_blocks(mcl_ctx *c, long long n)
 {
     long long b = (n + MCL_BLOCK - 1) / MCL_BLOCK;
     b = std::max<long long>(1, std::min<long long>(b, c->max_coop_blocks));
+    const char *e = getenv("MCL_COOP_BLOCKS");
+    if (e && atoi(e) > 0) b = std::min<long long>(b, atoi(e));
     return (unsigned)b;
 }
 
 static int reset_sync_if_aborted(mcl_ctx *c, const DevStats &st)
 {
     if (st.aborted) {
-        HIPCHK(c, hipMemsetAsync(c->gs.count, 0, 3 * sizeof(unsigned), c->stream));
+        HIPCHK(c, hipMemsetAsync(c->gs.gen, 0, MCL_SYNC_WORDS * sizeof(unsigned), c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         return fail(c, MCL_E_DEVICE, "grid barrier timed out");
     }
@@ -1067,19 +1071,11 @@ extern "C" int mcl_solve_velocity_pressure(mcl_ctx *c, mcl_stokes_stats *out)
         M.nmg = nm;
         M.cinv = c->cinv;
         M.nc = c->levels[0].neq;
-        size_t nv32 = std::max(L.neq, 1);
-        M.cg_r = c->cg32;
-        M.cg_z = c->cg32 + nv32;
-        M.cg_d = c->cg32 + 2 * nv32;
-        M.cg_q = c->cg32 + 3 * nv32;
-        M.cg_e = c->cg32 + 4 * nv32;
         M.rr = c->rr;
         M.inner_rtol = k.inner_rtol > 0 ? k.inner_rtol : 1e-8;
-        M.cg_rtol = 1e-4f;
-        M.max_refine = 10;
-        M.max_cg32 = 500;
-        M.pre = 2;
-        M.post = 2;
+        M.max_inner = 1000;
+        const char *sw = getenv("MCL_MG_SWEEPS");
+        M.pre = sw && atoi(sw) > 0 ? atoi(sw) : 2;
         M.omega = 4.0f / 3.0f;
         void *margs[] = {&M};
         HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_uzawa_mixed, dim3(P.gs.nblocks), dim3(MCL_BLOCK), margs,
@@ -1117,6 +1113,15 @@ extern "C" int mcl_solve_velocity_pressure(mcl_ctx *c, mcl_stokes_stats *out)
     hipEventDestroy(e1);
     rc = reset_sync_if_aborted(c, st);
     if (rc) return rc;
+#if MCL_PHASE_TIMERS
+    {
+        int khz = 0;
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device);
+        fprintf(stderr, "[mcl phases ms]");
+        for (int k = 0; k < MCL_PHASES; k++) fprintf(stderr, " %.2f", khz > 0 ? st.phase[k] / (double)khz : 0.0);
+        fprintf(stderr, "\n");
+    }
+#endif
     if (out) {
         out->uzawa_cycles = st.cycles;
         out->dv = st.dv;
@@ -1201,5 +1206,152 @@ extern "C" int mcl_update_temperature(mcl_ctx *c, mcl_heat_stats *out)
         out->model_time = mt;
         out->device_ms = ms;
     }
+    return 0;
+}
+
+// ------------------------------------------------------------ learned guess
+
+// Element-centre samples of the finest (uniform) level on an H x W image with
+// rows = z and columns = x: inputs (T, log10 eta), targets (u_x, u_z, p) / Ra.
+__global__ void k_net_sample(LevelDev L, int W, const double *__restrict__ T, const double *__restrict__ vis,
+                             const double *__restrict__ V, const double *__restrict__ p, double inv_Ra,
+                             float *__restrict__ x, float *__restrict__ t)
+{
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < L.nel; e += gridDim.x * blockDim.x) {
+        int g = L.node[L.ienl[e]];
+        int ix = g / L.gnoz, iz = g - ix * L.gnoz;
+        int pix = iz * W + ix;
+        double tc = 0.0, lc = 0.0, uc = 0.0, wc = 0.0;
+        for (int a = 0; a < 4; a++) {
+            int k = L.ienl[a * L.nel + e];
+            tc += 0.25 * T[k];
+            lc += 0.25 * log10(vis[k]);
+            uc += 0.25 * V[k];
+            wc += 0.25 * V[L.nno + k];
+        }
+        x[2 * pix] = (float)tc;
+        x[2 * pix + 1] = (float)lc;
+        if (t) {
+            t[3 * pix] = (float)(uc * inv_Ra);
+            t[3 * pix + 1] = (float)(wc * inv_Ra);
+            t[3 * pix + 2] = (float)(p[e] * inv_Ra);
+        }
+    }
+}
+
+// Network output -> (d, p): pressure per element, nodal velocity = mean of the
+// adjacent element-centre values, written to the free velocity equations.
+__global__ void k_net_unpack_p(LevelDev L, int W, const float *__restrict__ y, double scale, double *__restrict__ p)
+{
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < L.nel; e += gridDim.x * blockDim.x) {
+        int g = L.node[L.ienl[e]];
+        int ix = g / L.gnoz, iz = g - ix * L.gnoz;
+        p[e] = scale * (double)y[3 * (iz * W + ix) + 2];
+    }
+}
+
+__global__ void k_net_unpack_d(LevelDev L, int W, const float *__restrict__ y, double scale, double *__restrict__ d)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < L.nno; i += gridDim.x * blockDim.x) {
+        int cnt = L.p2n_cnt[i];
+        for (int k = 0; k < 2; k++) {
+            int eq = L.id[k * L.nno + i];
+            if (eq < 0) continue;
+            double s = 0.0;
+            for (int j = 0; j < cnt; j++) {
+                int e = L.p2n_el[j * L.nno + i];
+                int g = L.node[L.ienl[e]];
+                int ix = g / L.gnoz, iz = g - ix * L.gnoz;
+                s += (double)y[3 * (iz * W + ix) + k];
+            }
+            d[eq] = scale * s / cnt;
+        }
+    }
+}
+
+static int net_check(mcl_ctx *c, mcl_net *net, int *H, int *W, int *B)
+{
+    int rc = ready(c);
+    if (rc) return rc;
+    if (!net) return fail(c, MCL_E_ARG, "null network");
+    mcl_net_dims(net, H, W, B);
+    Level &L = c->levels.back();
+    if ((long long)(*H) * (*W) != L.nel || L.gnoz - 1 != *H || L.gnox - 1 != *W)
+        return fail(c, MCL_E_ARG, "network grid %dx%d does not match the uniform %dx%d mesh", *W, *H, L.gnox - 1,
+                    L.gnoz - 1);
+    for (int m : L.Mcode)
+        if ((m & 15) != L.L - 1) return fail(c, MCL_E_UNSUPPORTED, "learned guess needs a uniform mesh");
+    return 0;
+}
+
+extern "C" int mcl_net_capture(mcl_ctx *c, mcl_net *net)
+{
+    int H, W, B;
+    int rc = net_check(c, net, &H, &W, &B);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    Level &L = c->levels.back();
+    float *x = mcl_net_in(net), *t = mcl_net_tgt(net);  // batch slot 0 as staging
+    HIPCHK(c, hipStreamSynchronize(mcl_net_stream(net)));
+    hipLaunchKernelGGL(k_net_sample, dim3(grid_for(c, L.nel)), dim3(MCL_BLOCK), 0, c->stream, L.dev, W,
+                       (const double *)L.T, (const double *)L.vis, (const double *)L.V, (const double *)L.p,
+                       1.0 / c->ctl.Ra, x, t);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return mcl_net_add_snapshot(net, x, t) ? fail(c, MCL_E_DEVICE, "snapshot: %s", mcl_net_last_error(net)) : 0;
+}
+
+static int momentum_residual(mcl_ctx *c, const double *d, const double *p, double *res)
+{
+    Level &L = c->levels.back();
+    hipLaunchKernelGGL(k_residuals, dim3(grid_for(c, L.neq)), dim3(MCL_BLOCK), 0, c->stream, L.dev, c->rb, d, p);
+    hipLaunchKernelGGL(k_collect3, dim3(1), dim3(MCL_BLOCK), 0, c->stream, c->rb, 12, (int)grid_for(c, L.neq),
+                       c->resid);
+    HIPCHK(c, hipGetLastError());
+    double r[3];
+    HIPCHK(c, hipMemcpyAsync(r, c->resid, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *res = std::sqrt(r[0] + r[1]) / (r[2] > 0 ? std::sqrt(r[2]) : 1.0);
+    return 0;
+}
+
+extern "C" int mcl_stokes_guess(mcl_ctx *c, mcl_net *net, int *accepted, double *res_net, double *res_prev)
+{
+    int H, W, B;
+    int rc = net_check(c, net, &H, &W, &B);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    Level &L = c->levels.back();
+    float *x = mcl_net_in(net);
+    HIPCHK(c, hipStreamSynchronize(mcl_net_stream(net)));
+    hipLaunchKernelGGL(k_net_sample, dim3(grid_for(c, L.nel)), dim3(MCL_BLOCK), 0, c->stream, L.dev, W,
+                       (const double *)L.T, (const double *)L.vis, (const double *)L.V, (const double *)L.p,
+                       1.0 / c->ctl.Ra, x, (float *)nullptr);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (mcl_net_forward_device(net)) return fail(c, MCL_E_DEVICE, "network: %s", mcl_net_last_error(net));
+    HIPCHK(c, hipStreamSynchronize(mcl_net_stream(net)));
+    // candidate (d, p) in the Uzawa scratch space (u1 / Ahp regions)
+    double *dn = c->work, *pn = c->work + std::max(L.neq, 1);
+    double scale = (double)mcl_net_out_scale(net) * c->ctl.Ra;
+    HIPCHK(c, hipMemcpyAsync(dn, L.d, std::max(L.neq, 1) * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_net_unpack_d, dim3(grid_for(c, L.nno)), dim3(MCL_BLOCK), 0, c->stream, L.dev, W,
+                       (const float *)mcl_net_out(net), scale, dn);
+    hipLaunchKernelGGL(k_net_unpack_p, dim3(grid_for(c, L.nel)), dim3(MCL_BLOCK), 0, c->stream, L.dev, W,
+                       (const float *)mcl_net_out(net), scale, pn);
+    HIPCHK(c, hipGetLastError());
+    double rn = 0, rp = 0;
+    rc = momentum_residual(c, dn, pn, &rn);
+    if (rc) return rc;
+    rc = momentum_residual(c, L.d, L.p, &rp);
+    if (rc) return rc;
+    int acc = rn < rp ? 1 : 0;
+    if (acc) {
+        HIPCHK(c, hipMemcpyAsync(L.d, dn, std::max(L.neq, 1) * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(L.p, pn, L.nel * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (accepted) *accepted = acc;
+    if (res_net) *res_net = rn;
+    if (res_prev) *res_prev = rp;
     return 0;
 }

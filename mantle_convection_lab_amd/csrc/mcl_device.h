@@ -6,10 +6,14 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define MCL_BLOCK 256
+#ifndef MCL_BLOCK
+#define MCL_BLOCK 1024   // threads per workgroup of every kernel
+#endif
 #define MCL_WAVE 64
 #define MCL_MAX_LEVELS 20
 #define MCL_MAX_WG 2048
+// barrier words: gen at 0, abort at 32, arrive[MCL_MAX_WG] from 64
+#define MCL_SYNC_WORDS (64 + MCL_MAX_WG)
 #define MCL_SPIN_LIMIT (1u << 24)
 
 // Shape tables, uploaded once from the host (computed there with the
@@ -28,46 +32,63 @@ struct LevelSizes {
 };
 
 // ---------------------------------------------------------------- grid barrier
-// Sense/generation barrier.  Thread 0 of every workgroup arrives with an
-// agent-scope release, the last arrival resets the counter and bumps the
-// generation, the others spin on the generation with agent-scope acquires.
-// The spin is bounded: past MCL_SPIN_LIMIT polls the barrier sets *abort and
-// returns, and every caller then leaves its loops (all waves reach the end).
+// Flag barrier with a master workgroup (no atomic read-modify-write, so the
+// cost does not grow with serialised arrivals on one address).  Thread 0 of
+// every workgroup publishes the workgroup's writes with an agent-scope release
+// fence and then stores the new generation into its own arrive[] slot;
+// workgroup 0 polls all arrive[] slots (one slot per thread, coalesced), then
+// acquires, releases and stores the generation; the other workgroups poll the
+// generation and acquire.  Polls are relaxed agent-scope loads (no cache
+// maintenance per poll).  Every thread keeps the generation count itself
+// (all threads of the grid start from the same value), so the barrier is
+// monotone across launches: a slot or generation counts as reached when it is
+// at or past the target.  Spins are bounded: past MCL_SPIN_LIMIT polls the
+// barrier sets *abort and returns false, every spinner checks *abort, and
+// every caller then leaves its loops, so all waves reach the end of the
+// kernel.  After an abort the host zeroes gen, abort and arrive[].
 struct GridSync {
-    unsigned *count;
     unsigned *gen;
     unsigned *abort;
+    unsigned *arrive;   // [MCL_MAX_WG]
     unsigned nblocks;
 };
+
+__device__ __forceinline__ void spin_until(const unsigned *flag, unsigned target, unsigned *abort)
+{
+    unsigned spins = 0;
+    while ((int)(__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > MCL_SPIN_LIMIT) {
+            __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        if ((spins & 1023u) == 0 && __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    }
+}
 
 __device__ __forceinline__ bool grid_sync(const GridSync &gs, unsigned &my_gen)
 {
     __shared__ unsigned s_abort;
+    const unsigned target = my_gen + 1;
+    my_gen = target;
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned g = my_gen;
-        __threadfence();
-        unsigned arrived = __hip_atomic_fetch_add(gs.count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == gs.nblocks - 1) {
-            __hip_atomic_store(gs.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(gs.gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            unsigned spins = 0;
-            while (__hip_atomic_load(gs.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > MCL_SPIN_LIMIT) {
-                    __hip_atomic_store(gs.abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-                if ((spins & 1023u) == 0 &&
-                    __hip_atomic_load(gs.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                    break;
-            }
-        }
-        __threadfence();
-        my_gen = g + 1;
-        s_abort = __hip_atomic_load(gs.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(gs.arrive + blockIdx.x, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    if (blockIdx.x == 0) {
+        for (unsigned b = threadIdx.x; b < gs.nblocks; b += blockDim.x) spin_until(gs.arrive + b, target, gs.abort);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_store(gs.gen, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    } else if (threadIdx.x == 0) {
+        spin_until(gs.gen, target, gs.abort);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    if (threadIdx.x == 0) s_abort = __hip_atomic_load(gs.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     return s_abort == 0;
 }

@@ -53,10 +53,12 @@ struct LevelDev {
     const int *h_ni, *h_ci;          // [nh], [2][nh]
 };
 
+#define MCL_PHASES 8
 struct DevStats {
     int cycles, init_cycles, aborted, nan_flag;
     double dv, dp, dt;
     long long inner, spmv;
+    long long phase[MCL_PHASES];   // 100 MHz ticks per solver phase (builds with MCL_PHASE_TIMERS)
 };
 
 struct UzawaParams {
@@ -114,11 +116,10 @@ struct MixedParams {
     int nmg;
     const double *cinv;      // dense inverse of the coarsest K [nc][nc]
     int nc;
-    float *cg_r, *cg_z, *cg_d, *cg_q, *cg_e;   // FP32 inner CG vectors (finest rows)
-    double *rr;              // FP64 refinement residual
-    double inner_rtol;       // FP64 relative tolerance of each inner solve
-    float cg_rtol;           // relative tolerance of each FP32 CG sweep
-    int max_refine, max_cg32, pre, post;
+    double *rr;              // FP64 residual of the inner CG
+    double inner_rtol;       // relative residual tolerance of each inner solve
+    int max_inner;
+    int pre;                 // Jacobi sweeps per level on each leg of the V-cycle (>= 1)
     float omega;             // damped-Jacobi weight of the smoother
 };
 

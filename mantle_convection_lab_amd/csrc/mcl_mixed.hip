@@ -1,18 +1,39 @@
 // Mixed-precision Stokes solve of libmcl_hip.so (gfx950).
 //
 // The Uzawa outer iteration (Solver.c:14-249) runs in FP64 exactly as in the
-// FP64 path; only the inner velocity solves K u = b change:
-//   FP64 iterative refinement      r = b - K u (FP64 ELL),  u += |r| e
-//   around an FP32 CG              K32 e = r / |r| to a relative tolerance,
-//   preconditioned by a V-cycle    geometric multigrid over the quadtree levels
-//                                  (damped-Jacobi smoothing, restriction P^T,
-//                                  bilinear prolongation of the reference's
-//                                  interp maps, dense inverse on the coarsest
-//                                  grid); Jacobi when there is a single level.
+// FP64 path; only the inner velocity solves K u = b change: FP64 conjugate
+// gradients (FP64 ELL products, FP64 recurrences, compensated dot products)
+// preconditioned by an FP32 geometric-multigrid V-cycle over the quadtree
+// levels (damped-Jacobi smoothing, restriction P^T, prolongation P from the
+// reference's interp maps, dense inverse on the coarsest grid); Jacobi when
+// there is a single level.  The preconditioner is symmetric (equal pre- and
+// post-sweeps, R = P^T), so plain CG applies; FP32 only perturbs it.
 // Levels small enough for one workgroup are processed by workgroup 0 alone with
 // workgroup barriers; larger levels use the whole grid and grid barriers.
 #include "mcl_kernels.h"
 #include "mcl_ops.h"
+
+// Optional phase timers (build with -DMCL_PHASE_TIMERS=1): thread 0 of
+// workgroup 0 charges the 100 MHz wall clock since its previous stamp to a
+// phase; grid barriers keep every workgroup in step, so workgroup 0's split
+// is the grid's.  Phases: 0 d'/q pass, 1 x/r update (+ start), 2 V-cycle down
+// (grid levels), 3 workgroup-0 levels, 4 V-cycle up, 5 last sweep + r.z,
+// 6 Uzawa outer work.
+#if MCL_PHASE_TIMERS
+__shared__ long long mcl_ph[MCL_PHASES + 1];
+#define MCL_PHASE(k)                                                           \
+    do {                                                                       \
+        if (blockIdx.x == 0 && threadIdx.x == 0) {                             \
+            long long t_ = (long long)wall_clock64();                          \
+            mcl_ph[k] += t_ - mcl_ph[MCL_PHASES];                              \
+            mcl_ph[MCL_PHASES] = t_;                                           \
+        }                                                                      \
+    } while (0)
+#else
+#define MCL_PHASE(k) \
+    do {             \
+    } while (0)
+#endif
 
 // ------------------------------------------------------------ set-up kernels
 
@@ -83,6 +104,10 @@ __global__ void k_inject(int nc, const int *__restrict__ c2f, const double *__re
 }
 
 // --------------------------------------------------------------- level ops
+// Jacobi sweeps on a level alternate between two buffers: the iterate after k
+// sweeps from x = 0 is sweep_buf(A, k).  A V-cycle applies `pre` sweeps on the
+// way down and `pre` more on the way up (equal counts and R = P^T keep the
+// preconditioner symmetric), so every level's result is sweep_buf(A, 2 pre).
 
 struct Ex {
     int t, nt;
@@ -90,41 +115,55 @@ struct Ex {
 
 __device__ __forceinline__ float ell32(const MgLevelDev &A, int i, const float *__restrict__ x)
 {
+    // six entries per memory round trip; the sum keeps the slot order
     float s = 0.f;
-    for (int k = 0; k < A.WK; k++) s += A.val[k * A.n + i] * x[A.col[k * A.n + i]];
+    int k = 0;
+    for (; k + 6 <= A.WK; k += 6) {
+        int c[6];
+        float v[6];
+#pragma unroll
+        for (int u = 0; u < 6; u++) {
+            c[u] = A.col[(k + u) * A.n + i];
+            v[u] = A.val[(k + u) * A.n + i];
+        }
+#pragma unroll
+        for (int u = 0; u < 6; u++) s += v[u] * x[c[u]];
+    }
+    for (; k < A.WK; k++) s += A.val[k * A.n + i] * x[A.col[k * A.n + i]];
     return s;
 }
 
-// out = (first ? 0 : in) + w D^-1 (b - K in)
-__device__ __forceinline__ void jacobi_sweep(const MgLevelDev &A, const float *__restrict__ in, float *__restrict__ out,
-                                             float w, bool first, Ex ex)
+__device__ __forceinline__ float *sweep_buf(const MgLevelDev &A, int k) { return (k & 1) ? A.x : A.t; }
+
+// sweep k >= 2: x_k = x_{k-1} + w D^-1 (b - K x_{k-1})
+__device__ __forceinline__ void jacobi(const MgLevelDev &A, int k, float w, Ex ex)
 {
-    for (int i = ex.t; i < A.n; i += ex.nt) {
-        float bi = A.b[i];
-        out[i] = first ? w * A.dinv[i] * bi : in[i] + w * A.dinv[i] * (bi - ell32(A, i, in));
-    }
+    const float *__restrict__ in = sweep_buf(A, k - 1);
+    float *__restrict__ out = sweep_buf(A, k);
+    for (int i = ex.t; i < A.n; i += ex.nt) out[i] = in[i] + w * A.dinv[i] * (A.b[i] - ell32(A, i, in));
 }
 
-__device__ __forceinline__ void residual(const MgLevelDev &A, const float *__restrict__ x, Ex ex)
-{
-    for (int i = ex.t; i < A.n; i += ex.nt) A.r[i] = A.b[i] - ell32(A, i, x);
-}
-
-// b_c = R r_f with R = P^T (restriction arrays live on the coarse level)
-__device__ __forceinline__ void restrict_to(const MgLevelDev &C, const MgLevelDev &F, Ex ex)
+// Coarse right-hand side b_c = sum_k R_ck (b_u - (K x)_u) with the fine
+// residual recomputed per use (no fine residual vector, no barrier between
+// residual and restriction), fused with the coarse level's first sweep
+// x_c = w_c D^-1 b_c unless the coarse level is solved directly.
+__device__ __forceinline__ void restrict_residual(const MgLevelDev &C, const MgLevelDev &F, const float *__restrict__ xf,
+                                                  bool sweep, float wc, Ex ex)
 {
     for (int c = ex.t; c < C.n; c += ex.nt) {
         float s = 0.f;
         for (int k = 0; k < C.rw; k++) {
             int u = C.rcol[k * C.n + c];
-            if (u >= 0) s += C.rval[k * C.n + c] * F.r[u];
+            if (u >= 0) s += C.rval[k * C.n + c] * (F.b[u] - ell32(F, u, xf));
         }
         C.b[c] = s;
+        if (sweep) C.x[c] = wc * C.dinv[c] * s;
     }
 }
 
-// x_f += P x_c
-__device__ __forceinline__ void prolong_add(const MgLevelDev &F, const MgLevelDev &C, Ex ex)
+// x_f += P x_c: fine row u takes the mean of its pn[u] coarse rows
+__device__ __forceinline__ void prolong_add(const MgLevelDev &F, float *__restrict__ xf, const MgLevelDev &C,
+                                            const float *__restrict__ xc, Ex ex)
 {
     for (int u = ex.t; u < F.n; u += ex.nt) {
         int m = F.pn[u];
@@ -132,9 +171,9 @@ __device__ __forceinline__ void prolong_add(const MgLevelDev &F, const MgLevelDe
         float s = 0.f;
         for (int k = 0; k < m; k++) {
             int l = F.pcol[k * F.n + u];
-            if (l >= 0) s += C.x[l];
+            if (l >= 0) s += xc[l];
         }
-        F.x[u] += s / (float)m;
+        xf[u] += s / (float)m;
     }
 }
 
@@ -145,254 +184,272 @@ __device__ __forceinline__ bool lsync(bool grid, const GridSync &gs, unsigned &g
     return ok;
 }
 
-// Pre-smoothing from x = 0 with an even number of sweeps, result in A.x.
-__device__ __forceinline__ void smooth_pre(const MgLevelDev &A, float w, int sweeps, bool grid, Ex ex, const GridSync &gs,
-                           unsigned &gen, bool &ok)
+// Down the V-cycle from level l (its first sweep done and synchronised):
+// sweeps 2..pre, then the coarse right-hand side (and coarse first sweep).
+__device__ __forceinline__ void down_step(const MixedParams &P, const float *lam, int l, bool grid, Ex ex,
+                                          const GridSync &gs, unsigned &gen, bool &ok)
 {
-    float *cur = A.t, *nxt = A.x;
-    if (sweeps % 2 == 1) { cur = A.x; nxt = A.t; }
-    jacobi_sweep(A, nullptr, cur, w, true, ex);
+    const MgLevelDev &A = P.mg[l];
+    const float w = P.omega / lam[l];
+    for (int k = 2; k <= P.pre; k++) {
+        jacobi(A, k, w, ex);
+        ok = lsync(grid, gs, gen, ok);
+    }
+    restrict_residual(P.mg[l - 1], A, sweep_buf(A, P.pre), l - 1 > 0, P.omega / lam[l - 1], ex);
+}
+
+// Up the V-cycle at level l (level l-1 finished and synchronised): x += P x_c,
+// then the post-sweeps; with `last` false the final sweep is left to the
+// caller.  Ends synchronised.
+__device__ __forceinline__ void up_step(const MixedParams &P, const float *lam, int l, bool grid, bool last, Ex ex,
+                                        const GridSync &gs, unsigned &gen, bool &ok)
+{
+    const MgLevelDev &A = P.mg[l], &C = P.mg[l - 1];
+    const int kf = 2 * P.pre;
+    prolong_add(A, sweep_buf(A, P.pre), C, sweep_buf(C, kf), ex);
     ok = lsync(grid, gs, gen, ok);
-    for (int s = 1; s < sweeps; s++) {
-        jacobi_sweep(A, cur, nxt, w, false, ex);
+    const float w = P.omega / lam[l];
+    const int kend = last ? kf : kf - 1;
+    for (int k = P.pre + 1; k <= kend; k++) {
+        jacobi(A, k, w, ex);
         ok = lsync(grid, gs, gen, ok);
-        float *tmp = cur; cur = nxt; nxt = tmp;
     }
 }
 
-// Post-smoothing from the current A.x with an even number of sweeps.
-__device__ __forceinline__ void smooth_post(const MgLevelDev &A, float w, int sweeps, bool grid, Ex ex, const GridSync &gs,
-                            unsigned &gen, bool &ok)
+// Coarsest level: x = K0^-1 b by the dense FP64 inverse, four lanes per row
+// (one workgroup of MCL_BLOCK >= 4 n threads; n <= 64).
+__device__ __forceinline__ void coarse_solve(const MixedParams &P)
 {
-    float *cur = A.x, *nxt = A.t;
-    for (int s = 0; s < sweeps; s++) {
-        jacobi_sweep(A, cur, nxt, w, false, ex);
-        ok = lsync(grid, gs, gen, ok);
-        float *tmp = cur; cur = nxt; nxt = tmp;
-    }
+    const MgLevelDev &C = P.mg[0];
+    const int i = threadIdx.x >> 2, q = threadIdx.x & 3;
+    double s = 0.0;
+    if (i < C.n)
+        for (int j = q; j < C.n; j += 4) s += P.cinv[i * C.n + j] * (double)C.b[j];
+    s += __shfl_xor(s, 1, MCL_WAVE);
+    s += __shfl_xor(s, 2, MCL_WAVE);
+    if (q == 0 && i < C.n) sweep_buf(C, 2 * P.pre)[i] = (float)s;
 }
 
-// V-cycle on levels lo..top (all with the same execution mode), solving
-// A_top x = b_top; level 0 is solved with the dense inverse.
-__device__ __forceinline__ void vcycle_range(const MixedParams &P, const float *lam, int top, int lo, bool grid, Ex ex,
-                             const GridSync &gs, unsigned &gen, bool &ok)
-{
-    const MgLevelDev *mg = P.mg;
-    for (int l = top; l > lo; l--) {
-        const MgLevelDev &A = mg[l];
-        float w = P.omega / lam[l];
-        smooth_pre(A, w, P.pre, grid, ex, gs, gen, ok);
-        residual(A, A.x, ex);
-        ok = lsync(grid, gs, gen, ok);
-        restrict_to(mg[l - 1], A, ex);
-        ok = lsync(grid, gs, gen, ok);
-    }
-    if (lo == 0) {
-        const MgLevelDev &C = mg[0];
-        for (int i = ex.t; i < C.n; i += ex.nt) {
-            double s = 0.0;
-            for (int j = 0; j < C.n; j++) s += P.cinv[i * C.n + j] * (double)C.b[j];
-            C.x[i] = (float)s;
-        }
-    } else {
-        const MgLevelDev &A = mg[lo];
-        smooth_pre(A, P.omega / lam[lo], P.pre + P.post, grid, ex, gs, gen, ok);
-    }
-    ok = lsync(grid, gs, gen, ok);
-    for (int l = lo + 1; l <= top; l++) {
-        const MgLevelDev &A = mg[l];
-        prolong_add(A, mg[l - 1], ex);
-        ok = lsync(grid, gs, gen, ok);
-        smooth_post(A, P.omega / lam[l], P.post, grid, ex, gs, gen, ok);
-    }
-}
-
-// z = M r on the finest level: mg[top].b holds r, the result is in mg[top].x.
-__device__ __forceinline__ void vcycle(const MixedParams &P, const float *lam, const GridSync &gs, unsigned &gen, bool &ok)
+// The V-cycle after the finest level's right-hand side and first sweep are in
+// place and synchronised, up to (not including) the finest level's last
+// post-sweep, which the caller fuses with its dot product.  Levels with
+// `block` set (the coarse end of the hierarchy) run in workgroup 0 alone with
+// workgroup barriers; the others use the whole grid.
+__device__ __forceinline__ void vcycle_rest(const MixedParams &P, const float *lam, const GridSync &gs, unsigned &gen,
+                                            bool &ok)
 {
     const int top = P.nmg - 1;
-    int lb = 0;  // level 0 is always processed by one workgroup
+    int lb = 0;
     for (int l = 0; l <= top; l++)
         if (P.mg[l].block) lb = l;
-    Ex eg{(int)(blockIdx.x * blockDim.x + threadIdx.x), (int)(gridDim.x * blockDim.x)};
-    Ex eb{(int)threadIdx.x, (int)blockDim.x};
-    if (lb < top) {
-        // grid-wide levels above the block section; a V-cycle over them whose
-        // bottom (lb+1's coarse partner) is handled by workgroup 0
-        const MgLevelDev *mg = P.mg;
-        for (int l = top; l > lb; l--) {
-            const MgLevelDev &A = mg[l];
-            smooth_pre(A, P.omega / lam[l], P.pre, true, eg, gs, gen, ok);
-            residual(A, A.x, eg);
-            ok = lsync(true, gs, gen, ok);
-            restrict_to(mg[l - 1], A, eg);
-            ok = lsync(true, gs, gen, ok);
-        }
-        if (blockIdx.x == 0) vcycle_range(P, lam, lb, 0, false, eb, gs, gen, ok);
-        ok = lsync(true, gs, gen, ok);
-        for (int l = lb + 1; l <= top; l++) {
-            const MgLevelDev &A = mg[l];
-            prolong_add(A, mg[l - 1], eg);
-            ok = lsync(true, gs, gen, ok);
-            smooth_post(A, P.omega / lam[l], P.post, true, eg, gs, gen, ok);
-        }
-    } else {
-        if (blockIdx.x == 0) vcycle_range(P, lam, top, 0, false, eb, gs, gen, ok);
-        ok = lsync(true, gs, gen, ok);
+    const Ex eg{(int)(blockIdx.x * blockDim.x + threadIdx.x), (int)(gridDim.x * blockDim.x)};
+    const Ex eb{(int)threadIdx.x, (int)blockDim.x};
+    for (int l = top; l > lb; l--) {
+        if (l != top) ok = lsync(true, gs, gen, ok);
+        down_step(P, lam, l, true, eg, gs, gen, ok);
     }
+    if (lb < top) ok = lsync(true, gs, gen, ok);
+    MCL_PHASE(2);
+    if (blockIdx.x == 0) {
+        for (int l = lb; l > 0; l--) {
+            if (l != lb) __syncthreads();
+            down_step(P, lam, l, false, eb, gs, gen, ok);
+        }
+        __syncthreads();
+        coarse_solve(P);
+        __syncthreads();
+        for (int l = 1; l <= lb; l++) up_step(P, lam, l, false, l != top, eb, gs, gen, ok);
+    }
+    ok = lsync(true, gs, gen, ok);
+    MCL_PHASE(3);
+    for (int l = lb + 1; l <= top; l++) up_step(P, lam, l, true, l != top, eg, gs, gen, ok);
+    MCL_PHASE(4);
 }
 
-// ---------------------------------------------------------- FP32 inner CG
-
-// Solve K32 e = r (r in P.cg_r, overwritten), e in P.cg_e, to |r| <= rtol |r0|.
-// Returns the iteration count.
-__device__ __forceinline__ int cg32(const MixedParams &P, const float *lam, const GridSync &gs, unsigned &gen, bool &ok,
-                    dd (*lds)[MCL_BLOCK / MCL_WAVE])
+// Finest level's last post-sweep z = x_{2 pre}, fused with sum r_i z_i
+// (published to reduction slot `slot`).
+__device__ __forceinline__ void last_sweep_rz(const MixedParams &P, const float *lam, const double *__restrict__ r,
+                                              int slot, dd (*lds)[MCL_BLOCK / MCL_WAVE])
 {
     const MgLevelDev &A = P.mg[P.nmg - 1];
-    const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsize = gridDim.x * blockDim.x;
-    const int n = A.n;
-    const bool use_mg = P.nmg > 1;
-    float *r = P.cg_r, *e = P.cg_e, *d = P.cg_d, *q = P.cg_q;
-    float *z = use_mg ? A.x : P.cg_z;
-    // z = M r
-    if (use_mg) {
-        for (int i = gtid; i < n; i += gsize) A.b[i] = r[i];
-        vcycle(P, lam, gs, gen, ok);
-    } else {
-        for (int i = gtid; i < n; i += gsize) z[i] = A.dinv[i] * r[i];
+    const float w = P.omega / lam[P.nmg - 1];
+    const int kf = 2 * P.pre;
+    const float *__restrict__ in = sweep_buf(A, kf - 1);
+    float *__restrict__ out = sweep_buf(A, kf);
+    dd acc[1] = {dd_zero()};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+        float zi = in[i] + w * A.dinv[i] * (A.b[i] - ell32(A, i, in));
+        out[i] = zi;
+        acc[0] = dd_add_prod(acc[0], r[i], (double)zi);
     }
+    reduce_publish<1>(P.U.rb, slot, acc, lds);
+}
+
+// ------------------------------------------------- FP64 CG, FP32 V-cycle
+// Inner solve K x = b (x = 0 on entry when x_zero) by FP64 conjugate gradients
+// preconditioned with the FP32 multigrid V-cycle (Jacobi when there is a
+// single level): K x products, residuals and recurrences stay in FP64 with
+// compensated dot products; only z = M r is FP32.  Stops at
+// |b - K x| <= inner_rtol |b|.  Per iteration:
+//   (1) d' = z + beta d and q = K d' in one pass (q gathers z + beta d),
+//       with d'.q                                              | barrier
+//   (2) x += alpha d', r -= alpha q, r.r, finest b = r and the
+//       first Jacobi sweep                                     | barrier
+//   (3) rest of the V-cycle, last sweep fused with r.z          | barrier
+// Reduction slots: 0,1 (start), 2 (r.z), 12 (d.q), 13,14 (r.r, r.z).
+// Returns the iteration count.
+__device__ __forceinline__ int inner_mixed(const MixedParams &P, const float *lam, const double *__restrict__ b,
+                                           double *__restrict__ x, bool x_zero, const GridSync &gs,
+                                           unsigned &gen, bool &ok, dd (*lds)[MCL_BLOCK / MCL_WAVE],
+                                           long long &spmv)
+{
+    const LevelDev &L = P.U.L;
+    const bool use_mg = P.nmg > 1;
+    const MgLevelDev &A = P.mg[P.nmg - 1];
+    const float wtop = use_mg ? P.omega / lam[P.nmg - 1] : 0.f;
+    float *__restrict__ zf = use_mg ? sweep_buf(A, 2 * P.pre) : A.x;
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsize = gridDim.x * blockDim.x;
+    const int n = L.neq;
+    double *__restrict__ r = P.rr, *__restrict__ q = P.U.cg_q;
+    double *dold = P.U.cg_d, *dnew = P.U.cg_s;
+    // r = b - K x, then M r
     {
         dd acc[2] = {dd_zero(), dd_zero()};
         for (int i = gtid; i < n; i += gsize) {
-            float zi = z[i], ri = r[i];
-            d[i] = zi;
-            e[i] = 0.f;
-            acc[0] = dd_add_prod(acc[0], ri, zi);
-            acc[1] = dd_add_prod(acc[1], ri, ri);
+            double ri = x_zero ? b[i] : b[i] - ell_row(L.Kcol, L.Kval, L.Kncol[i], n, i, x);
+            if (x_zero) x[i] = 0.0;
+            r[i] = ri;
+            acc[0] = dd_add_prod(acc[0], ri, ri);
+            if (use_mg) {
+                float bi = (float)ri;
+                A.b[i] = bi;
+                A.x[i] = wtop * A.dinv[i] * bi;
+            } else {
+                float zi = (float)((double)A.dinv[i] * ri);
+                A.x[i] = zi;
+                acc[1] = dd_add_prod(acc[1], ri, (double)zi);
+            }
         }
-        reduce_publish<2>(P.U.rb, 2, acc, lds);
+        if (!x_zero) spmv++;
+        reduce_publish<2>(P.U.rb, 0, acc, lds);
     }
     ok = ok && grid_sync(gs, gen);
-    double delta, rr0;
+    double bn2, delta;
     {
         double o[2];
-        reduce_collect<2>(P.U.rb, 2, gridDim.x, o, lds);
-        delta = o[0];
-        rr0 = o[1];
+        reduce_collect<2>(P.U.rb, 0, gridDim.x, o, lds);
+        bn2 = o[0];
+        delta = o[1];
     }
-    const double stop2 = (double)P.cg_rtol * (double)P.cg_rtol * rr0;
+    MCL_PHASE(1);
+    if (!(bn2 > 0.0)) return 0;
+    const double stop2 = P.inner_rtol * P.inner_rtol * bn2;
+    if (use_mg) {
+        vcycle_rest(P, lam, gs, gen, ok);
+        last_sweep_rz(P, lam, r, 2, lds);
+        ok = ok && grid_sync(gs, gen);
+        double o[1];
+        reduce_collect<1>(P.U.rb, 2, gridDim.x, o, lds);
+        delta = o[0];
+        MCL_PHASE(5);
+    }
+    double beta = 0.0;
+    bool first = true;
     int it = 0;
-    while (ok && it < P.max_cg32 && rr0 > 0.0) {
+    while (ok && it < P.max_inner) {
+        double dq;
         {
             dd acc[1] = {dd_zero()};
             for (int i = gtid; i < n; i += gsize) {
-                float qi = ell32(A, i, d);
-                q[i] = qi;
-                acc[0] = dd_add_prod(acc[0], d[i], qi);
+                const int nc = L.Kncol[i];
+                double s = 0.0;
+                int k = 0;
+                for (; k + 6 <= nc; k += 6) {
+                    int c[6];
+                    double v[6];
+#pragma unroll
+                    for (int u = 0; u < 6; u++) {
+                        c[u] = L.Kcol[(k + u) * n + i];
+                        v[u] = L.Kval[(k + u) * n + i];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 6; u++) {
+                        double dj = (double)zf[c[u]];
+                        if (!first) dj += beta * dold[c[u]];
+                        s += v[u] * dj;
+                    }
+                }
+                for (; k < nc; k++) {
+                    const int j = L.Kcol[k * n + i];
+                    double dj = (double)zf[j];
+                    if (!first) dj += beta * dold[j];
+                    s += L.Kval[k * n + i] * dj;
+                }
+                double di = (double)zf[i];
+                if (!first) di += beta * dold[i];
+                dnew[i] = di;
+                q[i] = s;
+                acc[0] = dd_add_prod(acc[0], di, s);
             }
+            spmv++;
             reduce_publish<1>(P.U.rb, 12, acc, lds);
-        }
-        ok = ok && grid_sync(gs, gen);
-        double dq;
-        {
+            ok = ok && grid_sync(gs, gen);
             double o[1];
             reduce_collect<1>(P.U.rb, 12, gridDim.x, o, lds);
             dq = o[0];
+            MCL_PHASE(0);
         }
-        const float alpha = (float)(delta / dq);
-        double rr, rz;
+        if (!(dq > 0.0)) break;
+        const double alpha = delta / dq;
+        double rr, rz = 0.0;
         {
             dd acc[2] = {dd_zero(), dd_zero()};
             for (int i = gtid; i < n; i += gsize) {
-                e[i] += alpha * d[i];
-                float ri = r[i] - alpha * q[i];
+                x[i] += alpha * dnew[i];
+                double ri = r[i] - alpha * q[i];
                 r[i] = ri;
                 acc[0] = dd_add_prod(acc[0], ri, ri);
-                if (!use_mg) {
-                    float zi = A.dinv[i] * ri;
-                    z[i] = zi;
-                    acc[1] = dd_add_prod(acc[1], ri, zi);
+                if (use_mg) {
+                    float bi = (float)ri;
+                    A.b[i] = bi;
+                    A.x[i] = wtop * A.dinv[i] * bi;
                 } else {
-                    A.b[i] = ri;
+                    float zi = (float)((double)A.dinv[i] * ri);
+                    A.x[i] = zi;
+                    acc[1] = dd_add_prod(acc[1], ri, (double)zi);
                 }
             }
             reduce_publish<2>(P.U.rb, 13, acc, lds);
-        }
-        ok = ok && grid_sync(gs, gen);
-        {
+            ok = ok && grid_sync(gs, gen);
             double o[2];
             reduce_collect<2>(P.U.rb, 13, gridDim.x, o, lds);
             rr = o[0];
             rz = o[1];
+            MCL_PHASE(1);
         }
         it++;
         if (rr <= stop2) break;
         if (use_mg) {
-            vcycle(P, lam, gs, gen, ok);
-            dd acc[1] = {dd_zero()};
-            for (int i = gtid; i < n; i += gsize) acc[0] = dd_add_prod(acc[0], r[i], z[i]);
-            reduce_publish<1>(P.U.rb, 15, acc, lds);
+            vcycle_rest(P, lam, gs, gen, ok);
+            last_sweep_rz(P, lam, r, 2, lds);
             ok = ok && grid_sync(gs, gen);
             double o[1];
-            reduce_collect<1>(P.U.rb, 15, gridDim.x, o, lds);
+            reduce_collect<1>(P.U.rb, 2, gridDim.x, o, lds);
             rz = o[0];
+            MCL_PHASE(5);
         }
-        const float beta = (float)(rz / delta);
+        beta = rz / delta;
         delta = rz;
-        for (int i = gtid; i < n; i += gsize) d[i] = z[i] + beta * d[i];
+        first = false;
+        double *t = dold;
+        dold = dnew;
+        dnew = t;
+    }
+    if (L.nh > 0) {
+        constrain_v(L, x, gtid, gsize);
         ok = ok && grid_sync(gs, gen);
     }
     return it;
-}
-
-// Inner solve K x = b in FP64 by iterative refinement over cg32.
-__device__ __forceinline__ int inner_mixed(const MixedParams &P, const float *lam, const double *__restrict__ b,
-                           double *__restrict__ x, bool x_zero, const GridSync &gs, unsigned &gen, bool &ok,
-                           dd (*lds)[MCL_BLOCK / MCL_WAVE], long long &spmv)
-{
-    const LevelDev &L = P.U.L;
-    const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsize = gridDim.x * blockDim.x;
-    const int n = L.neq;
-    double bn = 0.0;
-    int total = 0;
-    for (int ref = 0; ok && ref <= P.max_refine; ref++) {
-        const bool zero = x_zero && ref == 0;
-        {
-            dd acc[2] = {dd_zero(), dd_zero()};
-            for (int i = gtid; i < n; i += gsize) {
-                double bi = b[i];
-                double ri = zero ? bi : bi - ell_row(L.Kcol, L.Kval, L.Kncol[i], n, i, x);
-                P.rr[i] = ri;
-                acc[0] = dd_add_prod(acc[0], ri, ri);
-                if (ref == 0) acc[1] = dd_add_prod(acc[1], bi, bi);
-            }
-            if (!zero) spmv++;
-            reduce_publish<2>(P.U.rb, 0, acc, lds);
-        }
-        ok = ok && grid_sync(gs, gen);
-        double rn;
-        {
-            double o[2];
-            reduce_collect<2>(P.U.rb, 0, gridDim.x, o, lds);
-            rn = sqrt(o[0]);
-            if (ref == 0) bn = sqrt(o[1]);
-        }
-        if (rn <= P.inner_rtol * bn || rn == 0.0 || ref == P.max_refine) break;
-        const double inv = 1.0 / rn;
-        for (int i = gtid; i < n; i += gsize) P.cg_r[i] = (float)(P.rr[i] * inv);
-        total += cg32(P, lam, gs, gen, ok, lds);
-        for (int i = gtid; i < n; i += gsize) {
-            double xi = zero ? 0.0 : x[i];
-            x[i] = xi + rn * (double)P.cg_e[i];
-        }
-        if (L.nh > 0) {
-            ok = ok && grid_sync(gs, gen);
-            constrain_v(L, x, gtid, gsize);
-        }
-        ok = ok && grid_sync(gs, gen);
-    }
-    return total;
 }
 
 // --------------------------------------------------------- persistent Uzawa
@@ -408,6 +465,12 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_uzawa_mixed(MixedParams P)
     unsigned gen = __hip_atomic_load(U.gs.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     bool ok = true;
     long long spmv = 0, inner = 0;
+#if MCL_PHASE_TIMERS
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < MCL_PHASES; k++) mcl_ph[k] = 0;
+        mcl_ph[MCL_PHASES] = (long long)wall_clock64();
+    }
+#endif
     for (int e = gtid; e < nel; e += gsize) {
         double s = 0.0;
         int lev = L.elev[e];
@@ -427,6 +490,7 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_uzawa_mixed(MixedParams P)
     }
     spmv++;
     ok = ok && grid_sync(U.gs, gen);
+    MCL_PHASE(6);
     inner += inner_mixed(P, lam, U.Ahv, U.u1, true, U.gs, gen, ok, lds, spmv);
     for (int i = gtid; i < neq; i += gsize) U.d[i] += U.u1[i];
     ok = ok && grid_sync(U.gs, gen);
@@ -461,7 +525,8 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_uzawa_mixed(MixedParams P)
             U.u1[i] = 0.0;
         }
         ok = ok && grid_sync(U.gs, gen);
-        inner += inner_mixed(P, lam, U.Ahv, U.u1, true, U.gs, gen, ok, lds, spmv);
+        MCL_PHASE(6);
+    inner += inner_mixed(P, lam, U.Ahv, U.u1, true, U.gs, gen, ok, lds, spmv);
         {
             dd acc[1] = {dd_zero()};
             for (int e = gtid; e < nel; e += gsize) {
@@ -518,4 +583,9 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_uzawa_mixed(MixedParams P)
         U.stats->aborted = ok ? 0 : 1;
         if (!(dv == dv) || !(dp == dp)) U.stats->nan_flag = 1;
     }
+    MCL_PHASE(6);
+#if MCL_PHASE_TIMERS
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (int k = 0; k < MCL_PHASES; k++) U.stats->phase[k] = mcl_ph[k];
+#endif
 }

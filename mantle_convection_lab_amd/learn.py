@@ -1,0 +1,132 @@
+"""Learned initial guess for the Stokes solve: network wrapper, snapshot
+generation from the solver, training, and the residual-gated guess.
+
+The network (U-Net, kernels in csrc/mcl_unet.hip) runs on the device through
+the C-ABI of include/mcl.h.  Snapshots are captured from the solver's own
+device fields after converged solves of seeded random runs (Rayleigh number
+log-uniform in [1e5, 1e7], viscosity contrast log-uniform in [1, 1e4]).
+"""
+import ctypes as C
+import math
+import os
+import time
+
+import numpy as np
+
+from . import _native as N
+from .model import Input, MclError, Model
+
+
+class Net:
+    """One U-Net instance on the device (H x W element-centre grid, batch B)."""
+
+    def __init__(self, H, W, batch=1, seed=0):
+        self.L = N.declare_net(N.hip_lib())
+        self.h = C.c_void_p()
+        rc = self.L.mcl_net_create(H, W, batch, seed, C.byref(self.h))
+        if rc:
+            raise MclError(f"mcl_net_create failed ({rc})")
+        self.H, self.W, self.B = H, W, batch
+        self.n = self.L.mcl_net_num_params(self.h)
+
+    def close(self):
+        if self.h:
+            self.L.mcl_net_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc):
+        if rc:
+            raise MclError(f"network error {rc}: {self.L.mcl_net_last_error(self.h).decode(errors='replace')}")
+
+    def params(self):
+        p = np.zeros(self.n, np.float32)
+        self._ck(self.L.mcl_net_get_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n
+        self._ck(self.L.mcl_net_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def capture(self, ctx):
+        rc = self.L.mcl_net_capture(ctx.h, self.h)
+        if rc:
+            raise MclError(f"capture failed ({rc}): {ctx.L.mcl_last_error(ctx.h).decode(errors='replace')}")
+
+    def dataset_size(self):
+        return self.L.mcl_net_dataset_size(self.h)
+
+    def train(self, epochs, lr=1e-3, seed=0):
+        losses = np.zeros(max(epochs, 1), np.float32)
+        self._ck(self.L.mcl_net_train(self.h, epochs, lr, seed, losses.ctypes.data_as(C.POINTER(C.c_float))))
+        return losses[:epochs]
+
+    def guess(self, ctx):
+        acc = C.c_int()
+        rn, rp = C.c_double(), C.c_double()
+        rc = self.L.mcl_stokes_guess(ctx.h, self.h, C.byref(acc), C.byref(rn), C.byref(rp))
+        if rc:
+            raise MclError(f"guess failed ({rc}): {ctx.L.mcl_last_error(ctx.h).decode(errors='replace')}")
+        return bool(acc.value), rn.value, rp.value
+
+    def save(self, path):
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        np.savez_compressed(path, params=self.params().astype(np.float32), H=self.H, W=self.W,
+                            scale=self.L.mcl_net_get_scale(self.h))
+
+    def load(self, path):
+        with np.load(path, allow_pickle=False) as z:
+            self.set_params(z["params"])
+            self._ck(self.L.mcl_net_set_scale(self.h, float(z["scale"])))
+
+
+def workload_keys(side, Ra, contrast, seed, **over):
+    """Keys of one convection instance: unit box, free slip, T=1 bottom,
+    eta = exp(-ln(contrast) T), seeded initial perturbation."""
+    level = int(round(math.log2(side))) + 1
+    k = dict(level=level, mg_min_level=2, nprox=1, nproz=1, model_length=1.0, model_height=1.0, Rayleigh=Ra,
+             A=math.log(contrast), refT=0.0, T_ini=4, T_seed=seed, T_mantle_perturb=0.01, Q=0.0, theta=0.5,
+             develop_mode=0, solver="cg", cg_precondition=1, accuracy=1e-8, citcom_solver_accu=1e-6,
+             max_citcom_cycle=2000, max_cg_cycle=200, schur_mode=1, precision=1, warm_start=1,
+             top_v_bc="SV", bot_v_bc="SV", lef_v_bc="VS", rig_v_bc="VS", write_output=0)
+    k.update(over)
+    return k
+
+
+def random_instance(rng, side):
+    Ra = 10 ** rng.uniform(5, 7)
+    contrast = 10 ** rng.uniform(0, 4)
+    return Ra, contrast
+
+
+def generate(net, side, runs, steps_per_run, seed=1, log=print):
+    """Run `runs` seeded random instances for `steps_per_run` steps each and
+    capture a snapshot after every converged solve."""
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    for r in range(runs):
+        Ra, ctr = random_instance(rng, side)
+        m = Model(Input.from_keys(workload_keys(side, Ra, ctr, seed=int(rng.integers(1, 2**40)))))
+        for s in range(steps_per_run):
+            c = m.ctx
+            c.buoyancy_force()
+            c.viscosity()
+            c.velocity_pressure_KGF_matrix()
+            st = c.solve_velocity_pressure()
+            if st.converged:
+                net.capture(c)
+            c.update_temperature()
+        m.ctx.close()
+        if log and (r + 1) % max(1, runs // 10) == 0:
+            log(f"[learn] run {r + 1}/{runs}: {net.dataset_size()} snapshots, {time.perf_counter() - t0:.1f} s")
+    return net.dataset_size()
+
+
+def default_weights_path(side):
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", f"unet_{side}.npz")

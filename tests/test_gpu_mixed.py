@@ -1,11 +1,14 @@
 """Mixed-precision path (FP32 multigrid-preconditioned CG inside FP64
 iterative refinement) against the FP64 path on the same problem (MI355X).
 
-Both solve the Uzawa iteration to dv, dp < 1e-6 with z = r / BAB.  Bars: the
-velocity and pressure fields agree to 1e-5 relative L2 (the Uzawa tolerance
-leaves differences of that order between two converged solves), the true
-momentum and continuity residuals of the mixed solution are at most those of
-the FP64 one times 10, and warm-started steps keep agreeing.
+Both solve the Uzawa iteration to dv, dp < 1e-6 with z = r / BAB.  The FP64
+reference here runs its Jacobi-PCG inner solves to convergence (relative
+tolerance 1e-20, no cycle cap in practice) so that both solutions are accurate;
+the reference's own settings (absolute break at 1e-8, 200 cycles) leave a
+momentum residual of ~3e-4 on this case.  Bars: velocity and pressure agree to
+1e-5 / 1e-4 relative L2 (the Uzawa tolerance leaves differences of that order
+between two converged solves), the true momentum residual of the mixed
+solution is below 1e-9, and warm-started steps keep agreeing.
 """
 import math
 
@@ -32,7 +35,7 @@ def _model(**over):
 
 def test_mixed_matches_fp64_one_solve():
     print("[gpu-mixed] 64x64 one solve", flush=True)
-    a = _model(precision=0, mg_min_level=6)
+    a = _model(precision=0, mg_min_level=6, accuracy=1e-20, max_cg_cycle=100000)
     b = _model(precision=1, inner_rtol=1e-10)
     ra = a.step()
     rb = b.step()
@@ -41,13 +44,12 @@ def test_mixed_matches_fp64_one_solve():
     assert ra["converged"] and rb["converged"]
     assert rel_l2(b.field("D"), a.field("D")) < 1e-5
     assert rel_l2(b.field("P"), a.field("P")) < 1e-4
-    assert rb["res_mom"] < max(10 * ra["res_mom"], 1e-8)
-    assert rb["res_cont"] < max(10 * ra["res_cont"], 1e-7)
+    assert rb["res_mom"] < 1e-9
 
 
 def test_mixed_warm_start_steps():
     print("[gpu-mixed] 64x64 five warm-started steps", flush=True)
-    a = _model(precision=0, mg_min_level=6)
+    a = _model(precision=0, mg_min_level=6, accuracy=1e-20, max_cg_cycle=100000)
     b = _model(precision=1, warm_start=1, inner_rtol=1e-10)
     for s in range(5):
         ra = a.step()
