@@ -164,7 +164,7 @@ typedef struct mcl_stokes_stats {
     double accuracy_used;
     double res_momentum;       /* |F - K d - G p| / |F| after the solve */
     double res_continuity;     /* |G^T d| / |F| after the solve */
-    double device_ms;          /* device time of the solve */
+    double device_ms;          /* device time of the persistent Uzawa kernel (HIP events on its stream) */
 } mcl_stokes_stats;
 
 typedef struct mcl_heat_stats {
@@ -207,11 +207,12 @@ int mcl_synchronize(mcl_ctx *ctx);
 
 /* ---------------------------------------------------------------------------
  * Learned initial guess (no counterpart in the reference: new in this library).
- * A U-Net maps temperature and log10 viscosity at the element centres of a
- * uniform H x W element mesh to u_x, u_z, p / Ra.  It is trained on the device
- * on snapshots of the solver's own fields; mcl_stokes_guess proposes (d, p)
- * for the next solve and keeps the current ones when the network's momentum
- * residual is larger.
+ * A U-Net maps the temperature anomaly (T minus its row mean, scaled to unit
+ * RMS s) and log10 viscosity at the element centres of a uniform H x W element
+ * mesh to (u_x, u_z, p - mean p) / (Ra s).  It is trained on the device on
+ * snapshots of the solver's own fields; mcl_stokes_guess proposes (d, p) for
+ * the next solve and keeps whichever of (network guess, current fields) has
+ * the smaller initial Schur residual |G^T K^-1 (F - G p)|.
  */
 typedef struct mcl_net mcl_net;
 
@@ -235,8 +236,10 @@ int mcl_net_capture(mcl_ctx *ctx, mcl_net *net);
 long long mcl_net_dataset_size(const mcl_net *net);
 /* epochs of shuffled mini-batches over the captured snapshots; losses[epoch] = mean batch loss */
 int mcl_net_train(mcl_net *net, int epochs, float lr, unsigned long long seed, float *losses);
-/* propose the network's (d, p) for the next solve of ctx; *accepted = 1 if it replaced the
- * current guess (smaller |F - K d - G p|); residual norms of both are returned if non-NULL */
+/* propose the network's (d, p) for the next solve of ctx (precision=1, warm_start=1, after
+ * mcl_velocity_pressure_KGF_matrix): for each candidate one inner solve gives
+ * d = K^-1 (F - G p) and r = |G^T d|; the candidate with the smaller r is left in the
+ * context's d, p (*accepted = 1: the network's); both r are returned if non-NULL */
 int mcl_stokes_guess(mcl_ctx *ctx, mcl_net *net, int *accepted, double *res_net, double *res_prev);
 
 #ifdef __cplusplus

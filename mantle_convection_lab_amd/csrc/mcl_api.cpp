@@ -42,6 +42,8 @@ struct Level {
     // device fields
     double *T = nullptr, *vis = nullptr, *buoy = nullptr, *V = nullptr, *d = nullptr, *p = nullptr;
     double *stress = nullptr, *es = nullptr, *area = nullptr, *avg = nullptr;
+    double *bterms = nullptr;  // [4][entries] terms of the buoyancy row averages
+    int bnent = 0;
     // multigrid (mixed precision): prolongation maps of this level (fine side)
     std::vector<int> interp_n, interp_eq;
     int *c2f = nullptr;                       // coarse node -> finest local node
@@ -83,6 +85,15 @@ struct mcl_ctx {
     double *cinv = nullptr;
     double *rr = nullptr;
     bool have_mg = false;
+    bool mixed_ready = false;        // FP32 hierarchy matches the assembled K
+    int mg_lb = 0;                   // highest multigrid level kept in LDS
+    int nbr_nb = 0;                  // workgroup count the neighbour lists were built for
+    int nbr_ok = 0;                  // lists usable (every list within MCL_BLOCK)
+    int *nbr_ptr = nullptr, *nbr_idx = nullptr;
+    double *gd = nullptr, *gp = nullptr;  // learned-guess candidate (d, p)
+    // learned guess: per-row partial sums and the sample scale [2 H + 1]
+    double *net_rows = nullptr;
+    int net_rows_h = 0;
     // heat work (finest level)
     double *tau = nullptr, *hK = nullptr, *hC = nullptr, *hA = nullptr, *hF = nullptr;
     double *hd = nullptr, *hdd = nullptr, *hd0 = nullptr, *hB = nullptr;
@@ -531,6 +542,8 @@ static int build_level(mcl_ctx *c, const mcl_mesh_view &v, Level &Lv)
     rc |= dev_upload(c, o, &cp, bent); Lv.brow_e = (int *)cp;
     rc |= dev_upload(c, o, &sp, bx); Lv.brow_xi = (signed char *)sp;
     rc |= dev_upload(c, o, &sp, bz); Lv.brow_eta = (signed char *)sp;
+    Lv.bnent = (int)bent.size();
+    rc |= dev_alloc(c, o, &Lv.bterms, 4 * std::max<size_t>(bent.size(), 1));
     rc |= dev_alloc(c, o, &Lv.T, nno);
     rc |= dev_alloc(c, o, &Lv.vis, nno);
     rc |= dev_alloc(c, o, &Lv.buoy, nno);
@@ -575,7 +588,8 @@ static int build_multigrid(mcl_ctx *c)
         MgLevelDev &M = mg[k];
         M.n = Lv.neq;
         M.WK = Lv.WK;
-        M.block = (k == 0 || Lv.neq <= block_rows_threshold()) ? 1 : 0;
+        M.block = 0;
+        M.lds_off = 0;
         M.col = Lv.dev.Kcol;
         M.ncol = Lv.dev.Kncol;
         M.val = Lv.val32;
@@ -598,6 +612,22 @@ static int build_multigrid(mcl_ctx *c)
             Lv.c2f = (int *)cp;
         }
     }
+    // coarse levels run per workgroup from LDS: level 0 (the dense solve) and
+    // the levels above it while each has at most block_rows_threshold() rows,
+    // their rows sum to at most MCL_MG_LDS_ROWS, and the level is not the finest
+    {
+        int off = 0;
+        for (int k = 0; k < nlev - 1; k++) {
+            const int n = c->levels[k].neq;
+            if (k > 0 && (n > block_rows_threshold() || off / 3 + n > MCL_MG_LDS_ROWS)) break;
+            if (off / 3 + n > MCL_MG_LDS_ROWS) break;
+            mg[k].block = 1;
+            mg[k].lds_off = off;
+            off += 3 * n;
+            c->mg_lb = k;
+        }
+    }
+    c->nbr_nb = 0;
     bool maps = nlev > 1;
     for (int k = 1; k < nlev; k++)
         if (c->levels[k].interp_n.empty()) maps = false;
@@ -670,6 +700,7 @@ static int build_multigrid(mcl_ctx *c)
     rc |= dev_alloc(c, c->owned, &c->rr, std::max(Fn.neq, 1));
     if (rc) return fail(c, MCL_E_DEVICE, "multigrid allocation failed");
     c->have_mg = maps;
+    c->mixed_ready = false;
     return 0;
 }
 
@@ -931,9 +962,13 @@ extern "C" int mcl_buoyancy_force(mcl_ctx *c)
         if (rc) return rc;
     }
     for (auto &L : c->levels) {
+        // the mixed path's coarse levels carry only K (their F is not used)
+        if (c->ctl.precision == MCL_PREC_MIXED && &L != &c->levels.back()) continue;
+        if (L.bnent > 0)
+            hipLaunchKernelGGL(k_buoy_terms, dim3(grid_for(c, L.bnent)), dim3(MCL_BLOCK), 0, c->stream, L.bnent, L.nno,
+                               L.brow_e, L.brow_xi, L.brow_eta, L.dev.ienl, L.nel, L.T, c->ctl.Ra, L.bterms);
         hipLaunchKernelGGL(k_buoy_rows, dim3((L.gnoz + MCL_BLOCK - 1) / MCL_BLOCK), dim3(MCL_BLOCK), 0, c->stream,
-                           L.gnoz, L.gnox, L.nno, L.brow_ptr, L.brow_e, L.brow_xi, L.brow_eta, L.dev.ienl, L.nel,
-                           L.T, c->ctl.Ra, L.avg);
+                           L.gnoz, L.gnox, L.brow_ptr, (const double *)L.bterms, L.avg);
         HIPCHK(c, hipGetLastError());
         hipLaunchKernelGGL(k_buoy_apply, dim3(grid_for(c, L.nno)), dim3(MCL_BLOCK), 0, c->stream, L.nno, L.gnoz,
                            L.dev.node, L.T, c->ctl.Ra, L.avg, L.buoy);
@@ -941,6 +976,8 @@ extern "C" int mcl_buoyancy_force(mcl_ctx *c)
     }
     return 0;
 }
+
+static int mixed_prepare(mcl_ctx *c);
 
 extern "C" int mcl_velocity_pressure_KGF_matrix(mcl_ctx *c)
 {
@@ -953,15 +990,20 @@ extern "C" int mcl_velocity_pressure_KGF_matrix(mcl_ctx *c)
                            (const double *)L.vis, (const double *)L.buoy);
         HIPCHK(c, hipGetLastError());
     }
+    c->mixed_ready = false;
+    if (c->ctl.precision == MCL_PREC_MIXED) return mixed_prepare(c);
     return 0;
 }
 
-// Workgroups of a persistent solve: one per 256 rows, at most what fits
-// co-resident, and at most MCL_COOP_BLOCKS if set (a grid barrier costs more
-// with more workgroups).
+// Workgroups of a persistent solve: one per MCL_ROWS_PER_THREAD * MCL_BLOCK
+// rows (a grid barrier costs more with more workgroups; a few rows per thread
+// keep enough loads in flight), at most what fits co-resident, and at most
+// MCL_COOP_BLOCKS if that is set.
+#define MCL_ROWS_PER_THREAD 4
 static unsigned coop_blocks(mcl_ctx *c, long long n)
 {
-    long long b = (n + MCL_BLOCK - 1) / MCL_BLOCK;
+    const long long per = (long long)MCL_ROWS_PER_THREAD * MCL_BLOCK;
+    long long b = (n + per - 1) / per;
     b = std::max<long long>(1, std::min<long long>(b, c->max_coop_blocks));
     const char *e = getenv("MCL_COOP_BLOCKS");
     if (e && atoi(e) > 0) b = std::min<long long>(b, atoi(e));
@@ -978,36 +1020,11 @@ static int reset_sync_if_aborted(mcl_ctx *c, const DevStats &st)
     return 0;
 }
 
-extern "C" int mcl_solve_velocity_pressure(mcl_ctx *c, mcl_stokes_stats *out)
+// Uzawa kernel parameters for the finest level's (d, p) and the work space.
+static UzawaParams uzawa_params(mcl_ctx *c, double accuracy)
 {
-    int rc = ready(c);
-    if (rc) return rc;
-    HIPCHK(c, hipSetDevice(c->device));
     Level &L = c->levels.back();
     const mcl_control &k = c->ctl;
-    // initial_d_p (Solver.c:251-278): d = 0 on every level; p = 0 unless a
-    // previous call exists, AMR is off and zero_P is off.
-    if (!k.warm_start) {
-        for (auto &Lv : c->levels) {
-            HIPCHK(c, hipMemsetAsync(Lv.d, 0, std::max(Lv.neq, 1) * sizeof(double), c->stream));
-            if (!(c->been_here_dp && k.AMR == 0 && k.zero_P == 0))
-                HIPCHK(c, hipMemsetAsync(Lv.p, 0, Lv.nel * sizeof(double), c->stream));
-        }
-    }
-    c->been_here_dp = true;
-    HIPCHK(c, hipMemsetAsync(c->stats, 0, sizeof(DevStats), c->stream));
-    double accuracy = k.accuracy;
-    if (k.auto_accuracy) {
-        // Solver.c:41-51: accuracy = citcom_accu * sqrt(F.F / neq)
-        hipLaunchKernelGGL(k_residuals, dim3(grid_for(c, L.neq)), dim3(MCL_BLOCK), 0, c->stream, L.dev, c->rb,
-                           (const double *)L.d, (const double *)L.p);
-        hipLaunchKernelGGL(k_collect3, dim3(1), dim3(MCL_BLOCK), 0, c->stream, c->rb, 12,
-                           (int)grid_for(c, L.neq), c->resid);
-        double r[3];
-        HIPCHK(c, hipMemcpyAsync(r, c->resid, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        accuracy = k.citcom_accu * std::sqrt(r[2] / L.neq);
-    }
     UzawaParams P{};
     P.L = L.dev;
     P.gs = c->gs;
@@ -1040,48 +1057,178 @@ extern "C" int mcl_solve_velocity_pressure(mcl_ctx *c, mcl_stokes_stats *out)
     P.check_v_only = k.check_v_only;
     P.schur = k.schur;
     P.gs.nblocks = coop_blocks(c, std::max(L.neq, L.nel));
+    return P;
+}
+
+// FP32 copies of K and smoother bounds per multigrid level and the dense
+// inverse of the coarsest K: once per assembly (mixed precision).
+static int mixed_prepare(mcl_ctx *c)
+{
+    const int nm = c->have_mg ? c->nlev : 1;
+    HIPCHK(c, hipMemsetAsync(c->lam_bits, 0, MCL_MAX_LEVELS * sizeof(unsigned), c->stream));
+    for (int m = 0; m < nm; m++) {
+        Level &Lv = c->have_mg ? c->levels[m] : c->levels.back();
+        if (Lv.neq == 0) continue;
+        hipLaunchKernelGGL(k_to_fp32, dim3(grid_for(c, Lv.neq)), dim3(MCL_BLOCK), 0, c->stream, Lv.neq, Lv.WK,
+                           (const double *)Lv.dev.Kval, (const double *)Lv.dev.Kdiag, Lv.val32, Lv.dinv32);
+        hipLaunchKernelGGL(k_gershgorin, dim3(grid_for(c, Lv.neq)), dim3(MCL_BLOCK), 0, c->stream, Lv.neq, Lv.WK,
+                           (const double *)Lv.dev.Kval, (const double *)Lv.dev.Kdiag, c->lam_bits + m);
+        HIPCHK(c, hipGetLastError());
+    }
+    if (c->have_mg) {
+        Level &C0 = c->levels[0];
+        hipLaunchKernelGGL(k_coarse_inverse, dim3(1), dim3(MCL_BLOCK), 0, c->stream, C0.neq, C0.WK, C0.dev.Kcol,
+                           C0.dev.Kncol, (const double *)C0.dev.Kval, c->cinv);
+        HIPCHK(c, hipGetLastError());
+    }
+    c->mixed_ready = true;
+    return 0;
+}
+
+// Neighbour lists for the point-to-point waits of the mixed V-cycle.  With
+// the contiguous ownership owner(l, i) = i / ceil(n_l / nb) of the kernel, a
+// workgroup's neighbours are the workgroups that own a column of one of its
+// rows on any grid level (all ELL slots), or a row coupled to one of its rows
+// by the prolongation between two grid levels (restriction is its transpose);
+// the relation is made symmetric, so every reader of a workgroup's rows is
+// its neighbour.  Lists longer than MCL_BLOCK turn the waits into grid
+// barriers.
+static int build_neighbours(mcl_ctx *c, int nb)
+{
+    const int top = c->nlev - 1, lb = c->mg_lb;
+    const int words = (nb + 63) / 64;
+    std::vector<uint64_t> bits((size_t)nb * words, 0);
+    auto add = [&](int a, int b) {
+        if (a == b) return;
+        bits[(size_t)a * words + (b >> 6)] |= 1ull << (b & 63);
+        bits[(size_t)b * words + (a >> 6)] |= 1ull << (a & 63);
+    };
+    for (int l = lb + 1; l <= top; l++) {
+        Level &A = c->levels[l];
+        const int n = A.neq;
+        if (n == 0) continue;
+        const int R = (n + nb - 1) / nb;
+        for (int s = 0; s < A.WK; s++)
+            for (int i = 0; i < n; i++) add(i / R, A.Kcol_h[(size_t)s * n + i] / R);
+        if (l - 1 > lb) {
+            Level &C = c->levels[l - 1];
+            const int Rc = (C.neq + nb - 1) / nb;
+            for (int a = 0; a < 4; a++)
+                for (int u = 0; u < n; u++) {
+                    if (A.interp_n[u] <= a) continue;
+                    int cc = A.interp_eq[(size_t)a * n + u];
+                    if (cc >= 0) add(u / R, cc / Rc);
+                }
+        }
+    }
+    std::vector<int> ptr(nb + 1, 0), idx;
+    int maxdeg = 0;
+    for (int b = 0; b < nb; b++) {
+        for (int w = 0; w < words; w++) {
+            uint64_t m = bits[(size_t)b * words + w];
+            while (m) {
+                int t = __builtin_ctzll(m);
+                idx.push_back(w * 64 + t);
+                m &= m - 1;
+            }
+        }
+        ptr[b + 1] = (int)idx.size();
+        maxdeg = std::max(maxdeg, ptr[b + 1] - ptr[b]);
+    }
+    for (void *q : {(void *)c->nbr_ptr, (void *)c->nbr_idx}) {
+        if (q) {
+            (void)hipFree(q);
+            c->owned.erase(std::remove(c->owned.begin(), c->owned.end(), q), c->owned.end());
+        }
+    }
+    c->nbr_ptr = c->nbr_idx = nullptr;
+    const int *cp;
+    if (dev_upload(c, c->owned, &cp, ptr)) return MCL_E_DEVICE;
+    c->nbr_ptr = (int *)cp;
+    if (idx.empty()) idx.push_back(0);
+    if (dev_upload(c, c->owned, &cp, idx)) return MCL_E_DEVICE;
+    c->nbr_idx = (int *)cp;
+    c->nbr_nb = nb;
+    const char *e = getenv("MCL_NBR_SYNC");
+    c->nbr_ok = (maxdeg <= MCL_BLOCK && !(e && atoi(e) == 0)) ? 1 : 0;
+    return 0;
+}
+
+static MixedParams mixed_params(mcl_ctx *c, const UzawaParams &P)
+{
+    const mcl_control &k = c->ctl;
+    MixedParams M{};
+    M.U = P;
+    M.mg = c->mg_dev;
+    M.lam = (const float *)c->lam_bits;
+    M.nmg = c->have_mg ? c->nlev : 1;
+    M.cinv = c->cinv;
+    M.nc = c->levels[0].neq;
+    M.rr = c->rr;
+    M.inner_rtol = k.inner_rtol > 0 ? k.inner_rtol : 1e-8;
+    M.max_inner = 1000;
+    M.probe = 0;
+    M.use_nbr = 0;
+    if (c->have_mg && (int)P.gs.nblocks != c->nbr_nb && build_neighbours(c, (int)P.gs.nblocks) == 0) {
+    }
+    if (c->have_mg && c->nbr_nb == (int)P.gs.nblocks) {
+        M.nbr_ptr = c->nbr_ptr;
+        M.nbr_idx = c->nbr_idx;
+        M.use_nbr = c->nbr_ok;
+    }
+    const char *sw = getenv("MCL_MG_SWEEPS");
+    M.pre = sw && atoi(sw) > 0 ? atoi(sw) : 2;
+    M.omega = 4.0f / 3.0f;
+    return M;
+}
+
+extern "C" int mcl_solve_velocity_pressure(mcl_ctx *c, mcl_stokes_stats *out)
+{
+    int rc = ready(c);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    Level &L = c->levels.back();
+    const mcl_control &k = c->ctl;
+    // initial_d_p (Solver.c:251-278): d = 0 on every level; p = 0 unless a
+    // previous call exists, AMR is off and zero_P is off.
+    if (!k.warm_start) {
+        for (auto &Lv : c->levels) {
+            HIPCHK(c, hipMemsetAsync(Lv.d, 0, std::max(Lv.neq, 1) * sizeof(double), c->stream));
+            if (!(c->been_here_dp && k.AMR == 0 && k.zero_P == 0))
+                HIPCHK(c, hipMemsetAsync(Lv.p, 0, Lv.nel * sizeof(double), c->stream));
+        }
+    }
+    c->been_here_dp = true;
+    HIPCHK(c, hipMemsetAsync(c->stats, 0, sizeof(DevStats), c->stream));
+    double accuracy = k.accuracy;
+    if (k.auto_accuracy) {
+        // Solver.c:41-51: accuracy = citcom_accu * sqrt(F.F / neq)
+        hipLaunchKernelGGL(k_residuals, dim3(grid_for(c, L.neq)), dim3(MCL_BLOCK), 0, c->stream, L.dev, c->rb,
+                           (const double *)L.d, (const double *)L.p);
+        hipLaunchKernelGGL(k_collect3, dim3(1), dim3(MCL_BLOCK), 0, c->stream, c->rb, 12,
+                           (int)grid_for(c, L.neq), c->resid);
+        double r[3];
+        HIPCHK(c, hipMemcpyAsync(r, c->resid, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        accuracy = k.citcom_accu * std::sqrt(r[2] / L.neq);
+    }
+    UzawaParams P = uzawa_params(c, accuracy);
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
-    HIPCHK(c, hipEventRecord(e0, c->stream));
     if (k.precision == MCL_PREC_MIXED) {
-        // FP32 copies of K and smoother bounds per multigrid level, dense
-        // inverse of the coarsest K, then the persistent mixed Uzawa kernel.
-        const int nm = c->have_mg ? c->nlev : 1;
-        HIPCHK(c, hipMemsetAsync(c->lam_bits, 0, MCL_MAX_LEVELS * sizeof(unsigned), c->stream));
-        for (int m = 0; m < nm; m++) {
-            Level &Lv = c->have_mg ? c->levels[m] : c->levels.back();
-            if (Lv.neq == 0) continue;
-            hipLaunchKernelGGL(k_to_fp32, dim3(grid_for(c, Lv.neq)), dim3(MCL_BLOCK), 0, c->stream, Lv.neq, Lv.WK,
-                               (const double *)Lv.dev.Kval, (const double *)Lv.dev.Kdiag, Lv.val32, Lv.dinv32);
-            hipLaunchKernelGGL(k_gershgorin, dim3(grid_for(c, Lv.neq)), dim3(MCL_BLOCK), 0, c->stream, Lv.neq,
-                               Lv.WK, (const double *)Lv.dev.Kval, (const double *)Lv.dev.Kdiag, c->lam_bits + m);
-            HIPCHK(c, hipGetLastError());
+        if (!c->mixed_ready) {
+            rc = mixed_prepare(c);
+            if (rc) return rc;
         }
-        if (c->have_mg) {
-            Level &C0 = c->levels[0];
-            hipLaunchKernelGGL(k_coarse_inverse, dim3(1), dim3(MCL_BLOCK), 0, c->stream, C0.neq, C0.WK,
-                               C0.dev.Kcol, C0.dev.Kncol, (const double *)C0.dev.Kval, c->cinv);
-            HIPCHK(c, hipGetLastError());
-        }
-        MixedParams M{};
-        M.U = P;
-        M.mg = c->mg_dev;
-        M.lam = (const float *)c->lam_bits;
-        M.nmg = nm;
-        M.cinv = c->cinv;
-        M.nc = c->levels[0].neq;
-        M.rr = c->rr;
-        M.inner_rtol = k.inner_rtol > 0 ? k.inner_rtol : 1e-8;
-        M.max_inner = 1000;
-        const char *sw = getenv("MCL_MG_SWEEPS");
-        M.pre = sw && atoi(sw) > 0 ? atoi(sw) : 2;
-        M.omega = 4.0f / 3.0f;
+        MixedParams M = mixed_params(c, P);
         void *margs[] = {&M};
+        HIPCHK(c, hipEventRecord(e0, c->stream));
         HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_uzawa_mixed, dim3(P.gs.nblocks), dim3(MCL_BLOCK), margs,
                                              0, c->stream));
     } else {
         void *args[] = {&P};
+        HIPCHK(c, hipEventRecord(e0, c->stream));
         HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_uzawa_fp64, dim3(P.gs.nblocks), dim3(MCL_BLOCK), args,
                                              0, c->stream));
     }
@@ -1239,10 +1386,80 @@ __global__ void k_net_sample(LevelDev L, int W, const double *__restrict__ T, co
     }
 }
 
+// Sample normalisation.  Stokes flow is linear in the buoyancy Ra (T - row
+// mean of T) for a fixed viscosity, so the network sees the temperature
+// anomaly scaled to unit RMS, a = (T - rowmean) / s, and learns the flow per
+// unit anomaly: targets (u, w, p - mean p) / (Ra s).  The guess is rescaled by
+// Ra s.  Pass 1 (one workgroup per image row): subtract the row mean of T and
+// accumulate the row's sum of squares and pressure sum.  Pass 2: every
+// workgroup sums the H row partials in the same order, so all get the same s
+// and pressure mean; it scales its pixels, and workgroup 0 stores s.
+__global__ void k_net_norm_rows(int H, int W, float *__restrict__ x, const float *__restrict__ t,
+                                double *__restrict__ rows)
+{
+    __shared__ double red[2][MCL_BLOCK / MCL_WAVE];
+    const int iz = blockIdx.x;
+    if (iz >= H) return;
+    double st = 0.0, sp = 0.0;
+    for (int ix = threadIdx.x; ix < W; ix += blockDim.x) {
+        st += (double)x[2 * (iz * W + ix)];
+        if (t) sp += (double)t[3 * (iz * W + ix) + 2];
+    }
+    for (int o = MCL_WAVE / 2; o > 0; o >>= 1) {
+        st += __shfl_xor(st, o, MCL_WAVE);
+        sp += __shfl_xor(sp, o, MCL_WAVE);
+    }
+    const int wv = threadIdx.x / MCL_WAVE, ln = threadIdx.x % MCL_WAVE;
+    if (ln == 0) { red[0][wv] = st; red[1][wv] = sp; }
+    __syncthreads();
+    st = 0.0; sp = 0.0;
+    for (int k = 0; k < (int)(blockDim.x / MCL_WAVE); k++) { st += red[0][k]; sp += red[1][k]; }
+    __syncthreads();
+    const double mean = st / W;
+    double sq = 0.0;
+    for (int ix = threadIdx.x; ix < W; ix += blockDim.x) {
+        float a = (float)((double)x[2 * (iz * W + ix)] - mean);
+        x[2 * (iz * W + ix)] = a;
+        sq += (double)a * a;
+    }
+    for (int o = MCL_WAVE / 2; o > 0; o >>= 1) sq += __shfl_xor(sq, o, MCL_WAVE);
+    if (ln == 0) red[0][wv] = sq;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double q = 0.0;
+        for (int k = 0; k < (int)(blockDim.x / MCL_WAVE); k++) q += red[0][k];
+        rows[iz] = q;
+        rows[H + iz] = sp;
+    }
+}
+
+__global__ void k_net_norm_apply(int H, int W, float *__restrict__ x, float *__restrict__ t,
+                                 double *__restrict__ rows)
+{
+    double q = 0.0, sp = 0.0;
+    for (int k = 0; k < H; k++) { q += rows[k]; sp += rows[H + k]; }
+    const double n = (double)H * W;
+    double sc = sqrt(q / n);
+    if (!(sc > 1e-300)) sc = 1e-300;
+    const double pm = sp / n;
+    const float inv = (float)(1.0 / sc);
+    for (int pix = blockIdx.x * blockDim.x + threadIdx.x; pix < H * W; pix += gridDim.x * blockDim.x) {
+        x[2 * pix] *= inv;
+        if (t) {
+            t[3 * pix] *= inv;
+            t[3 * pix + 1] *= inv;
+            t[3 * pix + 2] = (float)(((double)t[3 * pix + 2] - pm) / sc);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) rows[2 * H] = sc;
+}
+
 // Network output -> (d, p): pressure per element, nodal velocity = mean of the
 // adjacent element-centre values, written to the free velocity equations.
-__global__ void k_net_unpack_p(LevelDev L, int W, const float *__restrict__ y, double scale, double *__restrict__ p)
+__global__ void k_net_unpack_p(LevelDev L, int W, const float *__restrict__ y, double scale0,
+                               const double *__restrict__ sc, double *__restrict__ p)
 {
+    const double scale = scale0 * sc[0];
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < L.nel; e += gridDim.x * blockDim.x) {
         int g = L.node[L.ienl[e]];
         int ix = g / L.gnoz, iz = g - ix * L.gnoz;
@@ -1250,8 +1467,10 @@ __global__ void k_net_unpack_p(LevelDev L, int W, const float *__restrict__ y, d
     }
 }
 
-__global__ void k_net_unpack_d(LevelDev L, int W, const float *__restrict__ y, double scale, double *__restrict__ d)
+__global__ void k_net_unpack_d(LevelDev L, int W, const float *__restrict__ y, double scale0,
+                               const double *__restrict__ sc, double *__restrict__ d)
 {
+    const double scale = scale0 * sc[0];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < L.nno; i += gridDim.x * blockDim.x) {
         int cnt = L.p2n_cnt[i];
         for (int k = 0; k < 2; k++) {
@@ -1284,6 +1503,30 @@ static int net_check(mcl_ctx *c, mcl_net *net, int *H, int *W, int *B)
     return 0;
 }
 
+// sample the fields onto the network grid and normalise them (see above)
+static int net_sample(mcl_ctx *c, int H, int W, float *x, float *t)
+{
+    Level &L = c->levels.back();
+    if (c->net_rows_h != H) {
+        if (c->net_rows) {
+            (void)hipFree(c->net_rows);
+            c->owned.erase(std::remove(c->owned.begin(), c->owned.end(), (void *)c->net_rows), c->owned.end());
+            c->net_rows = nullptr;
+        }
+        if (dev_alloc(c, c->owned, &c->net_rows, 2 * (size_t)H + 1)) return MCL_E_DEVICE;
+        c->net_rows_h = H;
+    }
+    hipLaunchKernelGGL(k_net_sample, dim3(grid_for(c, L.nel)), dim3(MCL_BLOCK), 0, c->stream, L.dev, W,
+                       (const double *)L.T, (const double *)L.vis, (const double *)L.V, (const double *)L.p,
+                       1.0 / c->ctl.Ra, x, t);
+    hipLaunchKernelGGL(k_net_norm_rows, dim3(H), dim3(MCL_BLOCK), 0, c->stream, H, W, x, (const float *)t,
+                       c->net_rows);
+    hipLaunchKernelGGL(k_net_norm_apply, dim3(grid_for(c, (long long)H * W)), dim3(MCL_BLOCK), 0, c->stream, H, W,
+                       x, t, c->net_rows);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
 extern "C" int mcl_net_capture(mcl_ctx *c, mcl_net *net)
 {
     int H, W, B;
@@ -1293,25 +1536,33 @@ extern "C" int mcl_net_capture(mcl_ctx *c, mcl_net *net)
     Level &L = c->levels.back();
     float *x = mcl_net_in(net), *t = mcl_net_tgt(net);  // batch slot 0 as staging
     HIPCHK(c, hipStreamSynchronize(mcl_net_stream(net)));
-    hipLaunchKernelGGL(k_net_sample, dim3(grid_for(c, L.nel)), dim3(MCL_BLOCK), 0, c->stream, L.dev, W,
-                       (const double *)L.T, (const double *)L.vis, (const double *)L.V, (const double *)L.p,
-                       1.0 / c->ctl.Ra, x, t);
-    HIPCHK(c, hipGetLastError());
+    rc = net_sample(c, H, W, x, t);
+    if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)L;
     return mcl_net_add_snapshot(net, x, t) ? fail(c, MCL_E_DEVICE, "snapshot: %s", mcl_net_last_error(net)) : 0;
 }
 
-static int momentum_residual(mcl_ctx *c, const double *d, const double *p, double *res)
+// Initial Schur residual |G^T K^-1 (F - G p)| of the candidate (d, p): a
+// probe launch of the mixed Uzawa kernel runs the first inner solve (d becomes
+// K^-1 (F - G p), starting from the candidate's d) and stops.
+static int uzawa_probe(mcl_ctx *c, double *d, double *p, double *r1)
 {
-    Level &L = c->levels.back();
-    hipLaunchKernelGGL(k_residuals, dim3(grid_for(c, L.neq)), dim3(MCL_BLOCK), 0, c->stream, L.dev, c->rb, d, p);
-    hipLaunchKernelGGL(k_collect3, dim3(1), dim3(MCL_BLOCK), 0, c->stream, c->rb, 12, (int)grid_for(c, L.neq),
-                       c->resid);
-    HIPCHK(c, hipGetLastError());
-    double r[3];
-    HIPCHK(c, hipMemcpyAsync(r, c->resid, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->stats, 0, sizeof(DevStats), c->stream));
+    UzawaParams P = uzawa_params(c, c->ctl.accuracy);
+    P.d = d;
+    P.p = p;
+    MixedParams M = mixed_params(c, P);
+    M.probe = 1;
+    void *margs[] = {&M};
+    HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_uzawa_mixed, dim3(P.gs.nblocks), dim3(MCL_BLOCK), margs, 0,
+                                         c->stream));
+    DevStats st;
+    HIPCHK(c, hipMemcpyAsync(&st, c->stats, sizeof st, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    *res = std::sqrt(r[0] + r[1]) / (r[2] > 0 ? std::sqrt(r[2]) : 1.0);
+    int rc = reset_sync_if_aborted(c, st);
+    if (rc) return rc;
+    *r1 = std::sqrt(st.probe_r1);
     return 0;
 }
 
@@ -1320,34 +1571,44 @@ extern "C" int mcl_stokes_guess(mcl_ctx *c, mcl_net *net, int *accepted, double 
     int H, W, B;
     int rc = net_check(c, net, &H, &W, &B);
     if (rc) return rc;
+    if (c->ctl.precision != MCL_PREC_MIXED || !c->ctl.warm_start)
+        return fail(c, MCL_E_UNSUPPORTED, "the learned guess needs precision=1 and warm_start=1");
     HIPCHK(c, hipSetDevice(c->device));
     Level &L = c->levels.back();
+    if (!c->mixed_ready) {
+        rc = mixed_prepare(c);
+        if (rc) return rc;
+    }
+    if (!c->gd) {
+        if (dev_alloc(c, c->owned, &c->gd, std::max(L.neq, 1)) || dev_alloc(c, c->owned, &c->gp, L.nel))
+            return MCL_E_DEVICE;
+    }
     float *x = mcl_net_in(net);
     HIPCHK(c, hipStreamSynchronize(mcl_net_stream(net)));
-    hipLaunchKernelGGL(k_net_sample, dim3(grid_for(c, L.nel)), dim3(MCL_BLOCK), 0, c->stream, L.dev, W,
-                       (const double *)L.T, (const double *)L.vis, (const double *)L.V, (const double *)L.p,
-                       1.0 / c->ctl.Ra, x, (float *)nullptr);
+    rc = net_sample(c, H, W, x, nullptr);
+    if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (mcl_net_forward_device(net)) return fail(c, MCL_E_DEVICE, "network: %s", mcl_net_last_error(net));
     HIPCHK(c, hipStreamSynchronize(mcl_net_stream(net)));
-    // candidate (d, p) in the Uzawa scratch space (u1 / Ahp regions)
-    double *dn = c->work, *pn = c->work + std::max(L.neq, 1);
+    // candidate from the network; equations the network does not set keep the current d
     double scale = (double)mcl_net_out_scale(net) * c->ctl.Ra;
-    HIPCHK(c, hipMemcpyAsync(dn, L.d, std::max(L.neq, 1) * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->gd, L.d, std::max(L.neq, 1) * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     hipLaunchKernelGGL(k_net_unpack_d, dim3(grid_for(c, L.nno)), dim3(MCL_BLOCK), 0, c->stream, L.dev, W,
-                       (const float *)mcl_net_out(net), scale, dn);
+                       (const float *)mcl_net_out(net), scale, (const double *)(c->net_rows + 2 * H), c->gd);
     hipLaunchKernelGGL(k_net_unpack_p, dim3(grid_for(c, L.nel)), dim3(MCL_BLOCK), 0, c->stream, L.dev, W,
-                       (const float *)mcl_net_out(net), scale, pn);
+                       (const float *)mcl_net_out(net), scale, (const double *)(c->net_rows + 2 * H), c->gp);
     HIPCHK(c, hipGetLastError());
-    double rn = 0, rp = 0;
-    rc = momentum_residual(c, dn, pn, &rn);
+    // gate: the initial Schur residual of each candidate (the current fields
+    // are the previous step's solution, or zero on a cold start)
+    double rp = 0, rn = 0;
+    rc = uzawa_probe(c, L.d, L.p, &rp);
     if (rc) return rc;
-    rc = momentum_residual(c, L.d, L.p, &rp);
+    rc = uzawa_probe(c, c->gd, c->gp, &rn);
     if (rc) return rc;
     int acc = rn < rp ? 1 : 0;
     if (acc) {
-        HIPCHK(c, hipMemcpyAsync(L.d, dn, std::max(L.neq, 1) * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(L.p, pn, L.nel * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(L.d, c->gd, std::max(L.neq, 1) * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(L.p, c->gp, L.nel * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     if (accepted) *accepted = acc;

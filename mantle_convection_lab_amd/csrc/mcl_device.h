@@ -7,7 +7,7 @@
 #include <stdint.h>
 
 #ifndef MCL_BLOCK
-#define MCL_BLOCK 1024   // threads per workgroup of every kernel
+#define MCL_BLOCK 256   // threads per workgroup of every kernel
 #endif
 #define MCL_WAVE 64
 #define MCL_MAX_LEVELS 20
@@ -86,6 +86,31 @@ __device__ __forceinline__ bool grid_sync(const GridSync &gs, unsigned &my_gen)
         }
     } else if (threadIdx.x == 0) {
         spin_until(gs.gen, target, gs.abort);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    if (threadIdx.x == 0) s_abort = __hip_atomic_load(gs.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    return s_abort == 0;
+}
+
+// Point-to-point wait on the same generation counter: every workgroup
+// publishes its arrival as in grid_sync, then waits only for the workgroups of
+// its neighbour list (thread t polls the slot of its neighbour nbr, nbr < 0 for
+// none).  Used between phases whose data flow is confined to neighbours; the
+// counters stay consistent with grid_sync because every workgroup passes the
+// same sequence of waits.
+__device__ __forceinline__ bool nbr_sync(const GridSync &gs, unsigned &my_gen, int nbr)
+{
+    __shared__ unsigned s_abort;
+    const unsigned target = my_gen + 1;
+    my_gen = target;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(gs.arrive + blockIdx.x, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (nbr >= 0) {
+        spin_until(gs.arrive + nbr, target, gs.abort);
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     if (threadIdx.x == 0) s_abort = __hip_atomic_load(gs.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -43,31 +43,57 @@ __global__ void k_viscosity(int nno, const double *__restrict__ T, const double 
 // (elements ascending; within an element the interior points, then the top
 // row or right column points, then the corner node).  Each entry adds
 // N_a(xi, eta) * data[node_a] for a = 0..3 separately, as the reference does.
-__global__ void k_buoy_rows(int gnoz, int gnox, int nno, const int *__restrict__ ptr,
-                            const int *__restrict__ ent_e, const signed char *__restrict__ ent_xi,
-                            const signed char *__restrict__ ent_eta, const int *__restrict__ ienl, int nel,
-                            const double *__restrict__ T, double Ra, double *__restrict__ avg)
+// Row averages in two passes.  k_buoy_terms evaluates every term of every
+// row's ordered contribution list in parallel (the products are the
+// reference's, independent of the running sum); k_buoy_rows then adds each
+// row's terms in list order, one thread per row, with its loads unrolled, so
+// the sum rounds exactly as the reference's sequential loop.  Entries with a
+// negative element contribute one term (Ra T of the last node) and three
+// +0.0 terms, which leave the sum unchanged.
+__global__ void k_buoy_terms(int nent, int nno, const int *__restrict__ ent_e, const signed char *__restrict__ ent_xi,
+                             const signed char *__restrict__ ent_eta, const int *__restrict__ ienl, int nel,
+                             const double *__restrict__ T, double Ra, double *__restrict__ terms)
+{
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nent; t += gridDim.x * blockDim.x) {
+        int e = ent_e[t];
+        double v0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+        if (e < 0) {
+            v0 = Ra * T[nno - 1];
+        } else {
+            double xi = (double)ent_xi[t], eta = (double)ent_eta[t];
+            // Na values of Shape_function.c:180-197
+            double n0 = 0.25 * (1.0 - xi) * (1.0 - eta);
+            double n1 = 0.25 * (1.0 - xi) * (1.0 + eta);
+            double n2 = 0.25 * (1.0 + xi) * (1.0 + eta);
+            double n3 = 0.25 * (1.0 + xi) * (1.0 - eta);
+            v0 = n0 * (Ra * T[ienl[e]]);
+            v1 = n1 * (Ra * T[ienl[nel + e]]);
+            v2 = n2 * (Ra * T[ienl[2 * nel + e]]);
+            v3 = n3 * (Ra * T[ienl[3 * nel + e]]);
+        }
+        terms[4 * (size_t)t] = v0;
+        terms[4 * (size_t)t + 1] = v1;
+        terms[4 * (size_t)t + 2] = v2;
+        terms[4 * (size_t)t + 3] = v3;
+    }
+}
+
+__global__ void k_buoy_rows(int gnoz, int gnox, const int *__restrict__ ptr, const double *__restrict__ terms,
+                            double *__restrict__ avg)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= gnoz) return;
+    const int t0 = 4 * ptr[k], t1 = 4 * ptr[k + 1];
     double s = 0.0;
-    for (int t = ptr[k]; t < ptr[k + 1]; t++) {
-        int e = ent_e[t];
-        if (e < 0) {
-            s += Ra * T[nno - 1];
-            continue;
-        }
-        double xi = (double)ent_xi[t], eta = (double)ent_eta[t];
-        // Na values of Shape_function.c:180-197
-        double n0 = 0.25 * (1.0 - xi) * (1.0 - eta);
-        double n1 = 0.25 * (1.0 - xi) * (1.0 + eta);
-        double n2 = 0.25 * (1.0 + xi) * (1.0 + eta);
-        double n3 = 0.25 * (1.0 + xi) * (1.0 - eta);
-        s += n0 * (Ra * T[ienl[e]]);
-        s += n1 * (Ra * T[ienl[nel + e]]);
-        s += n2 * (Ra * T[ienl[2 * nel + e]]);
-        s += n3 * (Ra * T[ienl[3 * nel + e]]);
+    int t = t0;
+    for (; t + 8 <= t1; t += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = terms[t + u];
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += v[u];
     }
+    for (; t < t1; t++) s += terms[t];
     avg[k] = s / gnox;
 }
 

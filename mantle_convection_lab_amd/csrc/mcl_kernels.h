@@ -59,6 +59,7 @@ struct DevStats {
     double dv, dp, dt;
     long long inner, spmv;
     long long phase[MCL_PHASES];   // 100 MHz ticks per solver phase (builds with MCL_PHASE_TIMERS)
+    double probe_r1;               // |G^T d|^2 after the first inner solve (probe launches)
 };
 
 struct UzawaParams {
@@ -90,8 +91,14 @@ struct HeatParams {
 // the coarsest; the finest level shares its rows with the Stokes velocity
 // equations.  Levels with block != 0 are small enough to be processed by one
 // workgroup (LDS-speed barriers); the others are processed by the whole grid.
+// Rows (summed over levels) of the coarse multigrid levels that every
+// workgroup of the mixed solve keeps in LDS (three float vectors per row).
+#define MCL_MG_LDS_ROWS 1536
+
 struct MgLevelDev {
-    int n, WK, block;
+    int n, WK;
+    int block;               // coarse level run per workgroup from LDS
+    int lds_off;             // offset of its vectors in the LDS block (floats)
     const int *col;          // [WK][n] (the level's FP64 ELL pattern)
     const int *ncol;         // [n]
     float *val;              // [WK][n] FP32 copy of the level's K
@@ -119,6 +126,12 @@ struct MixedParams {
     double *rr;              // FP64 residual of the inner CG
     double inner_rtol;       // relative residual tolerance of each inner solve
     int max_inner;
+    int probe;               // 1: stop after the first inner solve and report |G^T d|^2
+    // point-to-point waits in the V-cycle: neighbour workgroups of each
+    // workgroup (CSR over workgroups, at most MCL_BLOCK each); use_nbr = 0
+    // makes every wait a grid barrier
+    const int *nbr_ptr, *nbr_idx;
+    int use_nbr;
     int pre;                 // Jacobi sweeps per level on each leg of the V-cycle (>= 1)
     float omega;             // damped-Jacobi weight of the smoother
 };
@@ -130,9 +143,10 @@ extern "C" hipError_t mcl_upload_shape(const ShapeTables *s);
 
 __global__ void k_viscosity(int nno, const double *T, const double *X, double A, double refT, double lm_jump,
                             double *vis);
-__global__ void k_buoy_rows(int gnoz, int gnox, int nno, const int *ptr, const int *ent_e,
-                            const signed char *ent_xi, const signed char *ent_eta, const int *ienl, int nel,
-                            const double *T, double Ra, double *avg);
+__global__ void k_buoy_terms(int nent, int nno, const int *ent_e, const signed char *ent_xi,
+                             const signed char *ent_eta, const int *ienl, int nel, const double *T, double Ra,
+                             double *terms);
+__global__ void k_buoy_rows(int gnoz, int gnox, const int *ptr, const double *terms, double *avg);
 __global__ void k_buoy_apply(int nno, int gnoz, const int *node, const double *T, double Ra, const double *avg,
                              double *buoy);
 __global__ void k_assemble_K(LevelDev L, const double *vis, const double *buoy);

@@ -109,25 +109,56 @@ __global__ void k_inject(int nc, const int *__restrict__ c2f, const double *__re
 // way down and `pre` more on the way up (equal counts and R = P^T keep the
 // preconditioner symmetric), so every level's result is sweep_buf(A, 2 pre).
 
+// Rows a thread visits on a level: begin, end, stride.
 struct Ex {
-    int t, nt;
+    int b, e, s;
 };
+
+// Contiguous row ownership of the V-cycle's grid levels: workgroup g owns rows
+// [g R, (g+1) R) with R = ceil(n / gridDim.x), so a workgroup's rows form a
+// compact patch of the Morton-ordered mesh and it exchanges data with few
+// others (the host's neighbour lists use the same formula).
+__device__ __forceinline__ Ex own_rows(int n)
+{
+    const int R = (n + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int b0 = (int)blockIdx.x * R;
+    return Ex{b0 + (int)threadIdx.x, min(b0 + R, n), (int)blockDim.x};
+}
+
+// all rows of a level, by one workgroup (the LDS levels)
+__device__ __forceinline__ Ex wg_rows(int n) { return Ex{(int)threadIdx.x, n, (int)blockDim.x}; }
 
 __device__ __forceinline__ float ell32(const MgLevelDev &A, int i, const float *__restrict__ x)
 {
-    // six entries per memory round trip; the sum keeps the slot order
+    // Rows of width 18 (the bilinear-quad velocity stencil) issue all their
+    // loads at once, other widths nine entries per round trip; the sum keeps
+    // the slot order.
     float s = 0.f;
-    int k = 0;
-    for (; k + 6 <= A.WK; k += 6) {
-        int c[6];
-        float v[6];
+    if (A.WK == 18) {
+        int c[18];
+        float v[18], g[18];
 #pragma unroll
-        for (int u = 0; u < 6; u++) {
+        for (int u = 0; u < 18; u++) {
+            c[u] = A.col[u * A.n + i];
+            v[u] = A.val[u * A.n + i];
+        }
+#pragma unroll
+        for (int u = 0; u < 18; u++) g[u] = x[c[u]];
+#pragma unroll
+        for (int u = 0; u < 18; u++) s += v[u] * g[u];
+        return s;
+    }
+    int k = 0;
+    for (; k + 9 <= A.WK; k += 9) {
+        int c[9];
+        float v[9];
+#pragma unroll
+        for (int u = 0; u < 9; u++) {
             c[u] = A.col[(k + u) * A.n + i];
             v[u] = A.val[(k + u) * A.n + i];
         }
 #pragma unroll
-        for (int u = 0; u < 6; u++) s += v[u] * x[c[u]];
+        for (int u = 0; u < 9; u++) s += v[u] * x[c[u]];
     }
     for (; k < A.WK; k++) s += A.val[k * A.n + i] * x[A.col[k * A.n + i]];
     return s;
@@ -140,21 +171,25 @@ __device__ __forceinline__ void jacobi(const MgLevelDev &A, int k, float w, Ex e
 {
     const float *__restrict__ in = sweep_buf(A, k - 1);
     float *__restrict__ out = sweep_buf(A, k);
-    for (int i = ex.t; i < A.n; i += ex.nt) out[i] = in[i] + w * A.dinv[i] * (A.b[i] - ell32(A, i, in));
+    for (int i = ex.b; i < ex.e; i += ex.s) out[i] = in[i] + w * A.dinv[i] * (A.b[i] - ell32(A, i, in));
 }
 
-// Coarse right-hand side b_c = sum_k R_ck (b_u - (K x)_u) with the fine
-// residual recomputed per use (no fine residual vector, no barrier between
-// residual and restriction), fused with the coarse level's first sweep
-// x_c = w_c D^-1 b_c unless the coarse level is solved directly.
-__device__ __forceinline__ void restrict_residual(const MgLevelDev &C, const MgLevelDev &F, const float *__restrict__ xf,
-                                                  bool sweep, float wc, Ex ex)
+// r = b - K x on a level
+__device__ __forceinline__ void residual(const MgLevelDev &A, const float *__restrict__ x, Ex ex)
 {
-    for (int c = ex.t; c < C.n; c += ex.nt) {
+    for (int i = ex.b; i < ex.e; i += ex.s) A.r[i] = A.b[i] - ell32(A, i, x);
+}
+
+// Coarse right-hand side b_c = sum_k R_ck r_u (R = P^T, arrays on the coarse
+// level), fused with the coarse level's first sweep x_c = w_c D^-1 b_c unless
+// the coarse level is solved directly.
+__device__ __forceinline__ void restrict_to(const MgLevelDev &C, const MgLevelDev &F, bool sweep, float wc, Ex ex)
+{
+    for (int c = ex.b; c < ex.e; c += ex.s) {
         float s = 0.f;
         for (int k = 0; k < C.rw; k++) {
             int u = C.rcol[k * C.n + c];
-            if (u >= 0) s += C.rval[k * C.n + c] * (F.b[u] - ell32(F, u, xf));
+            if (u >= 0) s += C.rval[k * C.n + c] * F.r[u];
         }
         C.b[c] = s;
         if (sweep) C.x[c] = wc * C.dinv[c] * s;
@@ -165,7 +200,7 @@ __device__ __forceinline__ void restrict_residual(const MgLevelDev &C, const MgL
 __device__ __forceinline__ void prolong_add(const MgLevelDev &F, float *__restrict__ xf, const MgLevelDev &C,
                                             const float *__restrict__ xc, Ex ex)
 {
-    for (int u = ex.t; u < F.n; u += ex.nt) {
+    for (int u = ex.b; u < ex.e; u += ex.s) {
         int m = F.pn[u];
         if (m == 0) continue;
         float s = 0.f;
@@ -177,50 +212,34 @@ __device__ __forceinline__ void prolong_add(const MgLevelDev &F, float *__restri
     }
 }
 
-__device__ __forceinline__ bool lsync(bool grid, const GridSync &gs, unsigned &gen, bool ok)
+__device__ __forceinline__ bool gsync(const GridSync &gs, unsigned &gen, bool ok) { return ok && grid_sync(gs, gen); }
+
+// Wait between two stencil phases of the V-cycle: only for the workgroups in
+// this workgroup's neighbour list (thread t polls neighbour t; nbr < 0: none),
+// or for the whole grid when the lists are off.
+__device__ __forceinline__ bool vsync(const GridSync &gs, unsigned &gen, bool ok, int nbr, bool use_nbr)
 {
-    if (grid) return ok && grid_sync(gs, gen);
-    __syncthreads();
-    return ok;
+    if (!use_nbr) return ok && grid_sync(gs, gen);
+    return ok && nbr_sync(gs, gen, nbr);
 }
 
-// Down the V-cycle from level l (its first sweep done and synchronised):
-// sweeps 2..pre, then the coarse right-hand side (and coarse first sweep).
-__device__ __forceinline__ void down_step(const MixedParams &P, const float *lam, int l, bool grid, Ex ex,
-                                          const GridSync &gs, unsigned &gen, bool &ok)
+// A coarse level as seen by one workgroup: its vectors live in the
+// workgroup's LDS block at lds_off (x, t, b; the residual reuses the sweep
+// buffer that is free after the pre-sweeps); the matrix stays in HBM.
+__device__ __forceinline__ MgLevelDev lds_level(const MgLevelDev &G, float *mgl, int pre)
 {
-    const MgLevelDev &A = P.mg[l];
-    const float w = P.omega / lam[l];
-    for (int k = 2; k <= P.pre; k++) {
-        jacobi(A, k, w, ex);
-        ok = lsync(grid, gs, gen, ok);
-    }
-    restrict_residual(P.mg[l - 1], A, sweep_buf(A, P.pre), l - 1 > 0, P.omega / lam[l - 1], ex);
-}
-
-// Up the V-cycle at level l (level l-1 finished and synchronised): x += P x_c,
-// then the post-sweeps; with `last` false the final sweep is left to the
-// caller.  Ends synchronised.
-__device__ __forceinline__ void up_step(const MixedParams &P, const float *lam, int l, bool grid, bool last, Ex ex,
-                                        const GridSync &gs, unsigned &gen, bool &ok)
-{
-    const MgLevelDev &A = P.mg[l], &C = P.mg[l - 1];
-    const int kf = 2 * P.pre;
-    prolong_add(A, sweep_buf(A, P.pre), C, sweep_buf(C, kf), ex);
-    ok = lsync(grid, gs, gen, ok);
-    const float w = P.omega / lam[l];
-    const int kend = last ? kf : kf - 1;
-    for (int k = P.pre + 1; k <= kend; k++) {
-        jacobi(A, k, w, ex);
-        ok = lsync(grid, gs, gen, ok);
-    }
+    MgLevelDev A = G;
+    A.x = mgl + G.lds_off;
+    A.t = A.x + G.n;
+    A.b = A.t + G.n;
+    A.r = sweep_buf(A, pre - 1);
+    return A;
 }
 
 // Coarsest level: x = K0^-1 b by the dense FP64 inverse, four lanes per row
-// (one workgroup of MCL_BLOCK >= 4 n threads; n <= 64).
-__device__ __forceinline__ void coarse_solve(const MixedParams &P)
+// (MCL_BLOCK >= 4 n threads; n <= 64).
+__device__ __forceinline__ void coarse_solve(const MixedParams &P, const MgLevelDev &C)
 {
-    const MgLevelDev &C = P.mg[0];
     const int i = threadIdx.x >> 2, q = threadIdx.x & 3;
     double s = 0.0;
     if (i < C.n)
@@ -232,37 +251,89 @@ __device__ __forceinline__ void coarse_solve(const MixedParams &P)
 
 // The V-cycle after the finest level's right-hand side and first sweep are in
 // place and synchronised, up to (not including) the finest level's last
-// post-sweep, which the caller fuses with its dot product.  Levels with
-// `block` set (the coarse end of the hierarchy) run in workgroup 0 alone with
-// workgroup barriers; the others use the whole grid.
-__device__ __forceinline__ void vcycle_rest(const MixedParams &P, const float *lam, const GridSync &gs, unsigned &gen,
-                                            bool &ok)
+// post-sweep, which the caller fuses with its dot product.
+//  * grid levels (lb+1 .. top): rows spread over the grid, grid barriers;
+//  * coarse levels (0 .. lb, `block` set): every workgroup runs them itself,
+//    identically, with its vectors in LDS and workgroup barriers only, so no
+//    workgroup waits on another and level lb+1 prolongates from its own LDS.
+__device__ __forceinline__ void vcycle_rest(const MixedParams &P, const float *lam, float *mgl, const GridSync &gs,
+                                            unsigned &gen, bool &ok, int nbr)
 {
-    const int top = P.nmg - 1;
+    const int top = P.nmg - 1, pre = P.pre, kf = 2 * P.pre;
+    const bool un = P.use_nbr != 0;
     int lb = 0;
-    for (int l = 0; l <= top; l++)
+    for (int l = 0; l < top; l++)
         if (P.mg[l].block) lb = l;
-    const Ex eg{(int)(blockIdx.x * blockDim.x + threadIdx.x), (int)(gridDim.x * blockDim.x)};
-    const Ex eb{(int)threadIdx.x, (int)blockDim.x};
     for (int l = top; l > lb; l--) {
-        if (l != top) ok = lsync(true, gs, gen, ok);
-        down_step(P, lam, l, true, eg, gs, gen, ok);
-    }
-    if (lb < top) ok = lsync(true, gs, gen, ok);
-    MCL_PHASE(2);
-    if (blockIdx.x == 0) {
-        for (int l = lb; l > 0; l--) {
-            if (l != lb) __syncthreads();
-            down_step(P, lam, l, false, eb, gs, gen, ok);
+        const MgLevelDev &A = P.mg[l];
+        const Ex ex = own_rows(A.n);
+        const float w = P.omega / lam[l];
+        for (int k = 2; k <= pre; k++) {
+            jacobi(A, k, w, ex);
+            ok = vsync(gs, gen, ok, nbr, un);
         }
-        __syncthreads();
-        coarse_solve(P);
-        __syncthreads();
-        for (int l = 1; l <= lb; l++) up_step(P, lam, l, false, l != top, eb, gs, gen, ok);
+        residual(A, sweep_buf(A, pre), ex);
+        if (l - 1 > lb) {
+            ok = vsync(gs, gen, ok, nbr, un);
+            restrict_to(P.mg[l - 1], A, true, P.omega / lam[l - 1], own_rows(P.mg[l - 1].n));
+            ok = vsync(gs, gen, ok, nbr, un);
+        } else {
+            // every workgroup restricts the whole residual into its LDS
+            ok = gsync(gs, gen, ok);
+        }
     }
-    ok = lsync(true, gs, gen, ok);
+    MCL_PHASE(2);
+    {
+        MgLevelDev B = lds_level(P.mg[lb], mgl, pre);
+        restrict_to(B, P.mg[lb + 1], lb > 0, P.omega / lam[lb], wg_rows(B.n));
+        __syncthreads();
+        for (int l = lb; l > 0; l--) {
+            const MgLevelDev A = lds_level(P.mg[l], mgl, pre);
+            const Ex ex = wg_rows(A.n);
+            const float w = P.omega / lam[l];
+            for (int k = 2; k <= pre; k++) {
+                jacobi(A, k, w, ex);
+                __syncthreads();
+            }
+            residual(A, sweep_buf(A, pre), ex);
+            __syncthreads();
+            const MgLevelDev C = lds_level(P.mg[l - 1], mgl, pre);
+            restrict_to(C, A, l - 1 > 0, P.omega / lam[l - 1], wg_rows(C.n));
+            __syncthreads();
+        }
+        coarse_solve(P, lds_level(P.mg[0], mgl, pre));
+        __syncthreads();
+        for (int l = 1; l <= lb; l++) {
+            const MgLevelDev A = lds_level(P.mg[l], mgl, pre);
+            const MgLevelDev C = lds_level(P.mg[l - 1], mgl, pre);
+            const Ex ex = wg_rows(A.n);
+            prolong_add(A, sweep_buf(A, pre), C, sweep_buf(C, kf), ex);
+            __syncthreads();
+            const float w = P.omega / lam[l];
+            for (int k = pre + 1; k <= kf; k++) {
+                jacobi(A, k, w, ex);
+                __syncthreads();
+            }
+        }
+    }
     MCL_PHASE(3);
-    for (int l = lb + 1; l <= top; l++) up_step(P, lam, l, true, l != top, eg, gs, gen, ok);
+    for (int l = lb + 1; l <= top; l++) {
+        const MgLevelDev &A = P.mg[l];
+        const Ex ex = own_rows(A.n);
+        if (l - 1 == lb) {
+            const MgLevelDev C = lds_level(P.mg[lb], mgl, pre);
+            prolong_add(A, sweep_buf(A, pre), C, sweep_buf(C, kf), ex);
+        } else {
+            prolong_add(A, sweep_buf(A, pre), P.mg[l - 1], sweep_buf(P.mg[l - 1], kf), ex);
+        }
+        ok = vsync(gs, gen, ok, nbr, un);
+        const float w = P.omega / lam[l];
+        const int kend = l == top ? kf - 1 : kf;
+        for (int k = pre + 1; k <= kend; k++) {
+            jacobi(A, k, w, ex);
+            ok = vsync(gs, gen, ok, nbr, un);
+        }
+    }
     MCL_PHASE(4);
 }
 
@@ -277,7 +348,8 @@ __device__ __forceinline__ void last_sweep_rz(const MixedParams &P, const float 
     const float *__restrict__ in = sweep_buf(A, kf - 1);
     float *__restrict__ out = sweep_buf(A, kf);
     dd acc[1] = {dd_zero()};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+    const Ex ex = own_rows(A.n);
+    for (int i = ex.b; i < ex.e; i += ex.s) {
         float zi = in[i] + w * A.dinv[i] * (A.b[i] - ell32(A, i, in));
         out[i] = zi;
         acc[0] = dd_add_prod(acc[0], r[i], (double)zi);
@@ -290,7 +362,7 @@ __device__ __forceinline__ void last_sweep_rz(const MixedParams &P, const float 
 // preconditioned with the FP32 multigrid V-cycle (Jacobi when there is a
 // single level): K x products, residuals and recurrences stay in FP64 with
 // compensated dot products; only z = M r is FP32.  Stops at
-// |b - K x| <= inner_rtol |b|.  Per iteration:
+// |b - K x|^2 <= max(inner_rtol^2 |b|^2, floor2).  Per iteration:
 //   (1) d' = z + beta d and q = K d' in one pass (q gathers z + beta d),
 //       with d'.q                                              | barrier
 //   (2) x += alpha d', r -= alpha q, r.r, finest b = r and the
@@ -301,7 +373,7 @@ __device__ __forceinline__ void last_sweep_rz(const MixedParams &P, const float 
 __device__ __forceinline__ int inner_mixed(const MixedParams &P, const float *lam, const double *__restrict__ b,
                                            double *__restrict__ x, bool x_zero, const GridSync &gs,
                                            unsigned &gen, bool &ok, dd (*lds)[MCL_BLOCK / MCL_WAVE],
-                                           long long &spmv)
+                                           float *mgl, long long &spmv, double floor2, int nbr)
 {
     const LevelDev &L = P.U.L;
     const bool use_mg = P.nmg > 1;
@@ -342,10 +414,16 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const float *la
         delta = o[1];
     }
     MCL_PHASE(1);
-    if (!(bn2 > 0.0)) return 0;
-    const double stop2 = P.inner_rtol * P.inner_rtol * bn2;
+    const double stop2 = fmax(P.inner_rtol * P.inner_rtol * bn2, floor2);
+    if (!(bn2 > stop2)) {
+        if (L.nh > 0) {
+            constrain_v(L, x, gtid, gsize);
+            ok = ok && grid_sync(gs, gen);
+        }
+        return 0;
+    }
     if (use_mg) {
-        vcycle_rest(P, lam, gs, gen, ok);
+        vcycle_rest(P, lam, mgl, gs, gen, ok, nbr);
         last_sweep_rz(P, lam, r, 2, lds);
         ok = ok && grid_sync(gs, gen);
         double o[1];
@@ -364,16 +442,16 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const float *la
                 const int nc = L.Kncol[i];
                 double s = 0.0;
                 int k = 0;
-                for (; k + 6 <= nc; k += 6) {
-                    int c[6];
-                    double v[6];
+                for (; k + 9 <= nc; k += 9) {
+                    int c[9];
+                    double v[9];
 #pragma unroll
-                    for (int u = 0; u < 6; u++) {
+                    for (int u = 0; u < 9; u++) {
                         c[u] = L.Kcol[(k + u) * n + i];
                         v[u] = L.Kval[(k + u) * n + i];
                     }
 #pragma unroll
-                    for (int u = 0; u < 6; u++) {
+                    for (int u = 0; u < 9; u++) {
                         double dj = (double)zf[c[u]];
                         if (!first) dj += beta * dold[c[u]];
                         s += v[u] * dj;
@@ -430,7 +508,7 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const float *la
         it++;
         if (rr <= stop2) break;
         if (use_mg) {
-            vcycle_rest(P, lam, gs, gen, ok);
+            vcycle_rest(P, lam, mgl, gs, gen, ok, nbr);
             last_sweep_rz(P, lam, r, 2, lds);
             ok = ok && grid_sync(gs, gen);
             double o[1];
@@ -457,6 +535,7 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const float *la
 __global__ void __launch_bounds__(MCL_BLOCK) k_uzawa_mixed(MixedParams P)
 {
     __shared__ dd lds[4][MCL_BLOCK / MCL_WAVE];
+    __shared__ float mgl[3 * MCL_MG_LDS_ROWS];
     const UzawaParams &U = P.U;
     const LevelDev &L = U.L;
     const float *lam = P.lam;
@@ -465,6 +544,12 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_uzawa_mixed(MixedParams P)
     unsigned gen = __hip_atomic_load(U.gs.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     bool ok = true;
     long long spmv = 0, inner = 0;
+    // this thread's neighbour for the V-cycle's point-to-point waits
+    int nbr = -1;
+    if (P.use_nbr) {
+        const int n0 = P.nbr_ptr[blockIdx.x], n1 = P.nbr_ptr[blockIdx.x + 1];
+        if ((int)threadIdx.x < n1 - n0) nbr = P.nbr_idx[n0 + threadIdx.x];
+    }
 #if MCL_PHASE_TIMERS
     if (threadIdx.x == 0) {
         for (int k = 0; k < MCL_PHASES; k++) mcl_ph[k] = 0;
@@ -482,19 +567,55 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_uzawa_mixed(MixedParams P)
         }
         U.BAB[e] = s;
     }
-    // momentum defect of the initial guess: Ah = F - G p - K d
-    for (int i = gtid; i < neq; i += gsize) {
-        double u = ell_row(L.Kcol, L.Kval, L.Kncol[i], neq, i, U.d);
-        U.Ahv[i] = L.F[i] - g_row(L, i, U.p) - u;
-        U.u1[i] = 0.0;
+    // momentum defect of the initial guess: Ah = F - G p - K d; |F|^2 gives
+    // the first inner solve an absolute floor inner_rtol |F| (a warm start's
+    // small defect is not reduced by another factor inner_rtol)
+    {
+        dd acc[1] = {dd_zero()};
+        for (int i = gtid; i < neq; i += gsize) {
+            double u = ell_row(L.Kcol, L.Kval, L.Kncol[i], neq, i, U.d);
+            double f = L.F[i];
+            U.Ahv[i] = f - g_row(L, i, U.p) - u;
+            U.u1[i] = 0.0;
+            acc[0] = dd_add_prod(acc[0], f, f);
+        }
+        reduce_publish<1>(U.rb, 5, acc, lds);
     }
     spmv++;
     ok = ok && grid_sync(U.gs, gen);
+    double ff;
+    {
+        double o[1];
+        reduce_collect<1>(U.rb, 5, gridDim.x, o, lds);
+        ff = o[0];
+    }
     MCL_PHASE(6);
-    inner += inner_mixed(P, lam, U.Ahv, U.u1, true, U.gs, gen, ok, lds, spmv);
+    inner += inner_mixed(P, lam, U.Ahv, U.u1, true, U.gs, gen, ok, lds, mgl, spmv,
+                         P.inner_rtol * P.inner_rtol * ff, nbr);
     for (int i = gtid; i < neq; i += gsize) U.d[i] += U.u1[i];
     ok = ok && grid_sync(U.gs, gen);
     double *r0 = U.r0, *r1 = U.r1, *r2 = U.r2, *z0 = U.z0, *z1 = U.z1, *s1 = U.s1, *s2 = U.s2;
+    if (P.probe) {
+        // initial Schur residual r1 = G^T K^-1 (F - G p) of this (d, p): the
+        // quantity the Uzawa iteration has to reduce
+        dd acc[1] = {dd_zero()};
+        for (int e = gtid; e < nel; e += gsize) {
+            double r = gt_elem(L, e, U.d);
+            acc[0] = dd_add_prod(acc[0], r, r);
+        }
+        reduce_publish<1>(U.rb, 4, acc, lds);
+        ok = ok && grid_sync(U.gs, gen);
+        double o[1];
+        reduce_collect<1>(U.rb, 4, gridDim.x, o, lds);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            U.stats->probe_r1 = o[0];
+            U.stats->cycles = 0;
+            U.stats->inner = inner;
+            U.stats->spmv = spmv;
+            U.stats->aborted = ok ? 0 : 1;
+        }
+        return;
+    }
     for (int e = gtid; e < nel; e += gsize) r1[e] = gt_elem(L, e, U.d);
     double dv = 1.0, dp = 1.0;
     int cycle = 0;
@@ -526,7 +647,7 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_uzawa_mixed(MixedParams P)
         }
         ok = ok && grid_sync(U.gs, gen);
         MCL_PHASE(6);
-    inner += inner_mixed(P, lam, U.Ahv, U.u1, true, U.gs, gen, ok, lds, spmv);
+        inner += inner_mixed(P, lam, U.Ahv, U.u1, true, U.gs, gen, ok, lds, mgl, spmv, 0.0, nbr);
         {
             dd acc[1] = {dd_zero()};
             for (int e = gtid; e < nel; e += gsize) {

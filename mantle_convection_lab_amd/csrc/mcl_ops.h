@@ -10,20 +10,20 @@
 __device__ __forceinline__ double ell_row(const int *__restrict__ col, const double *__restrict__ val, int ncol,
                                           int n, int row, const double *__restrict__ x)
 {
-    // loads issued six entries at a time (one memory round trip per chunk
+    // loads issued nine entries at a time (one memory round trip per chunk
     // instead of per entry); the sum keeps the ascending-column order
     double s = 0.0;
     int j = 0;
-    for (; j + 6 <= ncol; j += 6) {
-        int c[6];
-        double v[6];
+    for (; j + 9 <= ncol; j += 9) {
+        int c[9];
+        double v[9];
 #pragma unroll
-        for (int u = 0; u < 6; u++) {
+        for (int u = 0; u < 9; u++) {
             c[u] = col[(j + u) * n + row];
             v[u] = val[(j + u) * n + row];
         }
 #pragma unroll
-        for (int u = 0; u < 6; u++) s += v[u] * x[c[u]];
+        for (int u = 0; u < 9; u++) s += v[u] * x[c[u]];
     }
     for (; j < ncol; j++) s += val[j * n + row] * x[col[j * n + row]];
     return s;

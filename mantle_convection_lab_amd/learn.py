@@ -130,3 +130,78 @@ def generate(net, side, runs, steps_per_run, seed=1, log=print):
 
 def default_weights_path(side):
     return os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", f"unet_{side}.npz")
+
+
+def evaluate(weights, side, runs, steps, seed=7, log=print):
+    """Held-out seeded instances, each advanced twice from the same initial
+    state: with the network guess (gated against the previous step's fields)
+    and without it (previous step only; zero on the first step).  Returns
+    per-instance Uzawa iterations and solve times of both."""
+    rng = np.random.default_rng(seed)
+    net = Net(side, side, 1)
+    net.load(weights)
+    out = []
+    for r in range(runs):
+        Ra, ctr = random_instance(rng, side)
+        s = int(rng.integers(1, 2**40))
+        res = {}
+        for label, nn in (("net", net), ("plain", None)):
+            m = Model(Input.from_keys(workload_keys(side, Ra, ctr, seed=s)), net=nn)
+            recs = [m.step() for _ in range(steps)]
+            res[label] = dict(uzawa=[x["uzawa"] for x in recs], solve_ms=[x["solve_ms"] for x in recs],
+                              converged=[bool(x["converged"]) for x in recs],
+                              accepted=[bool(x.get("guess_accepted", False)) for x in recs])
+            m.ctx.close()
+        res.update(Ra=Ra, contrast=ctr, seed=s)
+        out.append(res)
+        if log:
+            log(f"[learn] held-out {r + 1}/{runs} Ra={Ra:.3g} contrast={ctr:.3g}: uzawa net {res['net']['uzawa']} "
+                f"plain {res['plain']['uzawa']}")
+    return out
+
+
+def main(argv=None):
+    """Generate snapshots with the solver, train the U-Net on the GPU with the
+    library's own kernels, save the weights, and evaluate on held-out runs.
+
+        python -m mantle_convection_lab_amd.learn --side 256 --runs 40 --steps 10 \
+            --epochs 60 --batch 16 --out mantle_convection_lab_amd/weights/unet_256.npz
+    """
+    import argparse
+    import json
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--side", type=int, default=256)
+    ap.add_argument("--runs", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--eval-runs", type=int, default=4)
+    ap.add_argument("--eval-steps", type=int, default=3)
+    args = ap.parse_args(argv)
+    out = args.out or default_weights_path(args.side)
+    net = Net(args.side, args.side, args.batch, seed=args.seed)
+    t0 = time.perf_counter()
+    n = generate(net, args.side, args.runs, args.steps, seed=args.seed)
+    print(f"[learn] {n} snapshots in {time.perf_counter() - t0:.1f} s", flush=True)
+    t0 = time.perf_counter()
+    losses = []
+    chunk = max(1, args.epochs // 10)
+    for e in range(0, args.epochs, chunk):
+        ls = net.train(min(chunk, args.epochs - e), lr=args.lr, seed=args.seed + e)
+        losses.extend(float(x) for x in ls)
+        print(f"[learn] epoch {len(losses)}/{args.epochs}: loss {losses[-1]:.4e} "
+              f"({time.perf_counter() - t0:.1f} s)", flush=True)
+    net.save(out)
+    net.close()
+    print(f"[learn] weights -> {out}", flush=True)
+    if args.eval_runs > 0:
+        ev = evaluate(out, args.side, args.eval_runs, args.eval_steps, seed=args.seed + 1000)
+        print(json.dumps({"losses": losses, "eval": ev}), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

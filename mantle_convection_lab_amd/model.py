@@ -247,8 +247,13 @@ class Context:
 class Model:
     """The reference's main loop (Main.c:28-69) over the device hot path."""
 
-    def __init__(self, inp, device=0, T0=None, ctrl_overrides=None):
+    def __init__(self, inp, device=0, T0=None, ctrl_overrides=None, net=None):
+        """net: optional learned-guess network (learn.Net) on the finest
+        grid; before every solve it proposes (d, p) and the candidate with the
+        smaller momentum residual (network or current fields) starts the
+        Uzawa iteration, which always runs to the same tolerance."""
         self.inp = inp
+        self.net = net
         self.mesh = Mesh(inp)
         self.ctx = Context(device)
         sizes = {}
@@ -276,6 +281,9 @@ class Model:
         c.buoyancy_force()
         c.viscosity()
         c.velocity_pressure_KGF_matrix()
+        guess = None
+        if self.net is not None:
+            guess = self.net.guess(c)
         st = c.solve_velocity_pressure()
         if self.inp.write_output and self.inp.save_spacing > 0 and self.timestep % self.inp.save_spacing == 0:
             self.write_output()
@@ -284,6 +292,8 @@ class Model:
                    inner=st.inner_iterations, solve_ms=st.device_ms, heat_cg=hs.cg_cycles, dt=hs.dt,
                    model_time=hs.model_time, res_mom=st.res_momentum, res_cont=st.res_continuity,
                    wall_s=time.perf_counter() - t0)
+        if guess is not None:
+            rec.update(guess_accepted=guess[0], guess_res_net=guess[1], guess_res_prev=guess[2])
         self.history.append(rec)
         self.timestep += 1
         return rec

@@ -1,5 +1,5 @@
-"""Mixed-precision path (FP32 multigrid-preconditioned CG inside FP64
-iterative refinement) against the FP64 path on the same problem (MI355X).
+"""Mixed-precision path (FP64 CG inner solves preconditioned by the FP32
+multigrid V-cycle) against the FP64 path on the same problem (MI355X).
 
 Both solve the Uzawa iteration to dv, dp < 1e-6 with z = r / BAB.  The FP64
 reference here runs its Jacobi-PCG inner solves to convergence (relative
