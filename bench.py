@@ -19,9 +19,10 @@ Paths (the Uzawa outer iteration and its stopping rule are the same in both):
                     [--path learned|mixed|fp64] [--no-cpu] [--copy-calibration]
 
 Prints one JSON line: value = converged Stokes solves per second of the primary
-path over its K timed steps (inputs already in HBM; wall clock between two
-device synchronisations), the plain FP64 path measured the same way in the same
-run over the same time steps, the roofline of the primary path's dominant kernel (the persistent Uzawa
+path over its K timed steps (time steps 0..K-1 of a fresh instance after W
+warmup steps on a throwaway instance; inputs already in HBM; wall clock between
+two device synchronisations), the plain FP64 path measured the same way in the
+same run over the same time steps, the roofline of the primary path's dominant kernel (the persistent Uzawa
 kernel) from its HIP-event durations, and a CPU baseline (the FP64 oracle on
 one core) on a bounded sample of the same workload.
 """
@@ -130,12 +131,19 @@ def run_path(path, steps, warmup, log):
         else:
             note = f"no trained weights at {os.path.relpath(wp, REPO)}: previous-step guess only"
             path = "mixed"
-    model = Model(Input.from_keys(path_keys(path)), device=0, net=net)
+    # warmup on a throwaway instance of the same workload (code paths, caches,
+    # allocations); the timed run then starts from step 0 on a fresh instance,
+    # so every path is timed over the same time steps
     warm = []
-    for i in range(warmup):
-        r = model.step()
-        warm.append(r)
-        log(f"[bench] {path} warmup {i}: uzawa={r['uzawa']} inner={r['inner']} solve={r['solve_ms']:.1f} ms")
+    if warmup > 0:
+        tw = Model(Input.from_keys(path_keys(path)), device=0, net=net)
+        for i in range(warmup):
+            r = tw.step()
+            warm.append(r)
+            log(f"[bench] {path} warmup {i}: uzawa={r['uzawa']} inner={r['inner']} solve={r['solve_ms']:.1f} ms")
+        tw.ctx.close()
+        del tw
+    model = Model(Input.from_keys(path_keys(path)), device=0, net=net)
     model.ctx.synchronize()
     t0 = time.perf_counter()
     recs = []
@@ -236,7 +244,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--fp64-steps", type=int, default=None,
                     help="timed steps of the plain FP64 comparison (default: --steps, after the same warmup)")
     ap.add_argument("--path", default="learned", choices=["learned", "mixed", "fp64"])
@@ -267,11 +275,11 @@ def main():
         fpr = run_path("fp64", fp_steps, args.warmup, log)
         fp = summary(fpr, fp_steps)
         fp["roofline"] = roofline(fpr, "k_uzawa_fp64")
-        first_cycles = (fpr["warm"] or fpr["recs"])[0]["uzawa"]
+        first_cycles = fpr["recs"][0]["uzawa"] if fpr["recs"] else None
         fpr["model"] = None
     elif args.path == "fp64":
         fp = ps
-        first_cycles = (prim["warm"] or prim["recs"])[0]["uzawa"]
+        first_cycles = prim["recs"][0]["uzawa"] if prim["recs"] else None
     out = {
         "metric": "Stokes solves/s at 1e-6 resid, 256² Ra=1e6: learned+FP32 vs plain FP64",
         "value": ps["solves_per_s"],

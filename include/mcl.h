@@ -224,9 +224,9 @@ long long mcl_net_num_params(const mcl_net *net);
 int mcl_net_get_params(mcl_net *net, float *host);
 int mcl_net_set_params(mcl_net *net, const float *host);
 int mcl_net_get_grads(mcl_net *net, float *host);
-/* output scale (RMS of the training targets, set by the first mcl_net_train) */
-double mcl_net_get_scale(const mcl_net *net);
-int mcl_net_set_scale(mcl_net *net, double scale);
+/* output scale of channel 0..2 (RMS of that training target, set by the first mcl_net_train) */
+double mcl_net_get_scale(const mcl_net *net, int channel);
+int mcl_net_set_scale(mcl_net *net, int channel, double scale);
 /* host arrays: x [B][H][W][2], y and target [B][H][W][3] */
 int mcl_net_forward(mcl_net *net, const float *x_host, float *y_host);
 /* one Adam step on the mean squared error (lr = 0: gradients only); loss is the batch MSE */
@@ -234,7 +234,8 @@ int mcl_net_train_step(mcl_net *net, const float *x_host, const float *t_host, f
 /* capture the context's current T, viscosity, velocity and pressure as a snapshot */
 int mcl_net_capture(mcl_ctx *ctx, mcl_net *net);
 long long mcl_net_dataset_size(const mcl_net *net);
-/* epochs of shuffled mini-batches over the captured snapshots; losses[epoch] = mean batch loss */
+/* epochs of shuffled mini-batches over the captured snapshots, each sample mirrored in x with
+ * probability 1/2; losses[epoch] = mean batch loss */
 int mcl_net_train(mcl_net *net, int epochs, float lr, unsigned long long seed, float *losses);
 /* propose the network's (d, p) for the next solve of ctx (precision=1, warm_start=1, after
  * mcl_velocity_pressure_KGF_matrix): for each candidate one inner solve gives

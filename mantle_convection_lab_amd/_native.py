@@ -145,6 +145,13 @@ def host_lib():
         L.mclh_build_mesh.argtypes = [C.POINTER(HostInput), C.POINTER(C.POINTER(HostMesh)), C.c_char_p, C.c_int]
         L.mclh_free_mesh.argtypes = [C.POINTER(HostMesh)]
         L.mclh_free_mesh.restype = None
+        L.mclh_build_mesh_depth.argtypes = [C.POINTER(HostInput), C.c_int, C.POINTER(C.POINTER(HostMesh)),
+                                            C.c_char_p, C.c_int]
+        L.mclh_adapt_mesh.argtypes = [C.POINTER(HostInput), C.POINTER(HostMesh), PD, C.c_int, C.c_int, C.c_double,
+                                      C.c_double, C.POINTER(C.POINTER(HostMesh)), C.POINTER(PD),
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, C.c_int]
+        L.mclh_free_array.argtypes = [C.c_void_p]
+        L.mclh_free_array.restype = None
         L.mclh_initial_temperature.argtypes = [C.POINTER(HostInput), C.POINTER(HostMesh), C.c_int, PD,
                                                C.c_char_p, C.c_int]
         L.mclh_fill_view.argtypes = [C.POINTER(HostMesh), C.c_int, C.POINTER(MeshView)]
@@ -180,7 +187,7 @@ def declare_net(L):
     L.mcl_net_dataset_size.restype = C.c_longlong
     L.mcl_net_train.argtypes = [vp, C.c_int, C.c_float, C.c_ulonglong, PF]
     L.mcl_stokes_guess.argtypes = [vp, vp, C.POINTER(C.c_int), PD, PD]
-    L.mcl_net_get_scale.argtypes = [vp]
+    L.mcl_net_get_scale.argtypes = [vp, C.c_int]
     L.mcl_net_get_scale.restype = C.c_double
-    L.mcl_net_set_scale.argtypes = [vp, C.c_double]
+    L.mcl_net_set_scale.argtypes = [vp, C.c_int, C.c_double]
     return L

@@ -534,6 +534,63 @@ static int build_level(mcl_ctx *c, const mcl_mesh_view &v, Level &Lv)
     rc |= dev_upload(c, o, &D.Tdiag_slot, Tdiag);
     D.nh = nh;
     D.h_serial = h_serial;
+    {
+        // Galerkin constraint maps (see LevelDev): expand each hanging
+        // equation to weighted free parents, then the transpose
+        std::vector<int> hidx(nno, -1);
+        for (int k = 0; k < nh; k++)
+            if (h_ni[k] >= 0) hidx[h_ni[k]] = k;
+        const size_t nq = std::max(neq, 1);
+        std::vector<int> hpn(nq, 0), hpi(4 * nq, 0);
+        std::vector<double> hpw(4 * nq, 0.0);
+        std::vector<std::vector<std::pair<int, double>>> ch(nq);
+        bool over = false;
+        for (int k = 0; k < nh; k++) {
+            for (int dd = 0; dd < 2; dd++) {
+                int eq = Lv.id[dd * nno + h_ni[k]];
+                if (eq < 0) continue;
+                std::vector<std::pair<int, double>> par;
+                std::vector<std::pair<int, double>> todo = {{h_ci[k], 0.5}, {h_ci[nh + k], 0.5}};
+                int guard = 0;
+                while (!todo.empty() && guard++ < 64) {
+                    auto [node, w] = todo.back();
+                    todo.pop_back();
+                    if (hidx[node] >= 0) {
+                        int kk = hidx[node];
+                        todo.push_back({h_ci[kk], 0.5 * w});
+                        todo.push_back({h_ci[nh + kk], 0.5 * w});
+                    } else {
+                        int pe = Lv.id[dd * nno + node];
+                        if (pe >= 0) par.push_back({pe, w});
+                    }
+                }
+                if (par.size() > 4) { over = true; continue; }
+                hpn[eq] = (int)par.size();
+                if (par.empty()) hpn[eq] = -1;  // all parents fixed: the value is 0
+                for (size_t t = 0; t < par.size(); t++) {
+                    hpi[t * nq + eq] = par[t].first;
+                    hpw[t * nq + eq] = par[t].second;
+                    ch[par[t].first].push_back({eq, par[t].second});
+                }
+            }
+        }
+        if (over) return fail(c, MCL_E_UNSUPPORTED, "hanging node with more than 4 free parents");
+        std::vector<int> cptr(nq + 1, 0), ci;
+        std::vector<double> cw;
+        for (size_t i = 0; i < nq; i++) {
+            for (auto &pw : ch[i]) { ci.push_back(pw.first); cw.push_back(pw.second); }
+            cptr[i + 1] = (int)ci.size();
+        }
+        if (ci.empty()) { ci.push_back(0); cw.push_back(0.0); }
+        int rc2 = 0;
+        rc2 |= dev_upload(c, Lv.owned, &D.hp_n, hpn);
+        rc2 |= dev_upload(c, Lv.owned, &D.hp_i, hpi);
+        rc2 |= dev_upload(c, Lv.owned, &D.hp_w, hpw);
+        rc2 |= dev_upload(c, Lv.owned, &D.ch_ptr, cptr);
+        rc2 |= dev_upload(c, Lv.owned, &D.ch_i, ci);
+        rc2 |= dev_upload(c, Lv.owned, &D.ch_w, cw);
+        if (rc2) return MCL_E_DEVICE;
+    }
     rc |= dev_upload(c, o, &D.h_ni, h_ni);
     rc |= dev_upload(c, o, &D.h_ci, h_ci);
     const int *cp;
@@ -1467,10 +1524,10 @@ __global__ void k_net_unpack_p(LevelDev L, int W, const float *__restrict__ y, d
     }
 }
 
-__global__ void k_net_unpack_d(LevelDev L, int W, const float *__restrict__ y, double scale0,
+__global__ void k_net_unpack_d(LevelDev L, int W, const float *__restrict__ y, double su, double sw,
                                const double *__restrict__ sc, double *__restrict__ d)
 {
-    const double scale = scale0 * sc[0];
+    const double scale[2] = {su * sc[0], sw * sc[0]};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < L.nno; i += gridDim.x * blockDim.x) {
         int cnt = L.p2n_cnt[i];
         for (int k = 0; k < 2; k++) {
@@ -1483,7 +1540,7 @@ __global__ void k_net_unpack_d(LevelDev L, int W, const float *__restrict__ y, d
                 int ix = g / L.gnoz, iz = g - ix * L.gnoz;
                 s += (double)y[3 * (iz * W + ix) + k];
             }
-            d[eq] = scale * s / cnt;
+            d[eq] = scale[k] * s / cnt;
         }
     }
 }
@@ -1555,7 +1612,7 @@ static int uzawa_probe(mcl_ctx *c, double *d, double *p, double *r1)
     MixedParams M = mixed_params(c, P);
     M.probe = 1;
     void *margs[] = {&M};
-    HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_uzawa_mixed, dim3(P.gs.nblocks), dim3(MCL_BLOCK), margs, 0,
+    HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_uzawa_probe, dim3(P.gs.nblocks), dim3(MCL_BLOCK), margs, 0,
                                          c->stream));
     DevStats st;
     HIPCHK(c, hipMemcpyAsync(&st, c->stats, sizeof st, hipMemcpyDeviceToHost, c->stream));
@@ -1591,12 +1648,14 @@ extern "C" int mcl_stokes_guess(mcl_ctx *c, mcl_net *net, int *accepted, double 
     if (mcl_net_forward_device(net)) return fail(c, MCL_E_DEVICE, "network: %s", mcl_net_last_error(net));
     HIPCHK(c, hipStreamSynchronize(mcl_net_stream(net)));
     // candidate from the network; equations the network does not set keep the current d
-    double scale = (double)mcl_net_out_scale(net) * c->ctl.Ra;
+    const double Ra = c->ctl.Ra;
     HIPCHK(c, hipMemcpyAsync(c->gd, L.d, std::max(L.neq, 1) * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     hipLaunchKernelGGL(k_net_unpack_d, dim3(grid_for(c, L.nno)), dim3(MCL_BLOCK), 0, c->stream, L.dev, W,
-                       (const float *)mcl_net_out(net), scale, (const double *)(c->net_rows + 2 * H), c->gd);
+                       (const float *)mcl_net_out(net), Ra * mcl_net_out_scale(net, 0), Ra * mcl_net_out_scale(net, 1),
+                       (const double *)(c->net_rows + 2 * H), c->gd);
     hipLaunchKernelGGL(k_net_unpack_p, dim3(grid_for(c, L.nel)), dim3(MCL_BLOCK), 0, c->stream, L.dev, W,
-                       (const float *)mcl_net_out(net), scale, (const double *)(c->net_rows + 2 * H), c->gp);
+                       (const float *)mcl_net_out(net), Ra * mcl_net_out_scale(net, 2),
+                       (const double *)(c->net_rows + 2 * H), c->gp);
     HIPCHK(c, hipGetLastError());
     // gate: the initial Schur residual of each candidate (the current fields
     // are the previous step's solution, or zero on a cold start)

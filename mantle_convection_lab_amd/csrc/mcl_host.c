@@ -674,15 +674,14 @@ void mclh_free_mesh(mclh_mesh *m)
  * build_project_conn / setup_project_comm_frame (:1051-1350). */
 static void build_mg_maps(mclh_level *lo, mclh_level *up);
 
-int mclh_build_mesh(const mclh_input *in, mclh_mesh **out, char *err, int errlen)
+/* All levels from the finest level's sorted leaf codes (takes ownership of
+ * top.v): project down level by level (build_quad_tree, Tree.c:67-93), build
+ * each level's structures and the multigrid transfer maps. */
+static int mesh_from_codes(const mclh_input *in, ivec top, mclh_mesh **out, char *err, int errlen)
 {
     mclh_mesh *m;
-    ivec top = {0};
-    int L = in->max_level, ip, k, np = in->npx * in->npz;
+    int L = in->max_level, k;
     *out = NULL;
-    if (L < 2 || L > 16) return herr(err, errlen, "level must be in [2,16]");
-    if (in->min_level < 0 || in->min_level >= L) return herr(err, errlen, "bad mg_min_level");
-    if (np < 1 || in->npx < 1 || in->npz < 1) return herr(err, errlen, "bad nprox/nproz");
     m = calloc(1, sizeof *m);
     m->L = L;
     m->min_level = in->min_level;
@@ -695,16 +694,8 @@ int mclh_build_mesh(const mclh_input *in, mclh_mesh **out, char *err, int errlen
     }
     m->gnox = (1 << (L - 1)) * in->npx + 1;
     m->gnoz = (1 << (L - 1)) * in->npz + 1;
-    for (ip = 0; ip < np; ip++)
-        if (rank_codes(L, in->dev_mode, ip, in->npz, &top)) {
-            free(top.v);
-            free(m);
-            return herr(err, errlen, "develop_mode %d not available", in->dev_mode);
-        }
-    qsort(top.v, top.n, sizeof(int), icmp);
     m->lev = calloc(m->nlev, sizeof(mclh_level));
     {
-        /* finest level first, then project down (build_quad_tree, Tree.c:67-93) */
         int **codes = calloc(m->nlev, sizeof(int *));
         int *ncodes = calloc(m->nlev, sizeof(int));
         codes[m->nlev - 1] = top.v;
@@ -735,6 +726,225 @@ int mclh_build_mesh(const mclh_input *in, mclh_mesh **out, char *err, int errlen
     *out = m;
     return 0;
 }
+
+static int check_dims(const mclh_input *in, char *err, int errlen)
+{
+    int L = in->max_level, np = in->npx * in->npz;
+    if (L < 2 || L > 16) return herr(err, errlen, "level must be in [2,16]");
+    if (in->min_level < 0 || in->min_level >= L) return herr(err, errlen, "bad mg_min_level");
+    if (np < 1 || in->npx < 1 || in->npz < 1) return herr(err, errlen, "bad nprox/nproz");
+    return 0;
+}
+
+int mclh_build_mesh(const mclh_input *in, mclh_mesh **out, char *err, int errlen)
+{
+    ivec top = {0};
+    int L = in->max_level, ip, np = in->npx * in->npz;
+    *out = NULL;
+    if (check_dims(in, err, errlen)) return 1;
+    for (ip = 0; ip < np; ip++)
+        if (rank_codes(L, in->dev_mode, ip, in->npz, &top)) {
+            free(top.v);
+            return herr(err, errlen, "develop_mode %d not available", in->dev_mode);
+        }
+    qsort(top.v, top.n, sizeof(int), icmp);
+    return mesh_from_codes(in, top, out, err, errlen);
+}
+
+/* ----------------------------------------------------------- adaptivity
+ * The quadtree as a depth grid: the depth of the leaf covering each finest
+ * cell, cells [ix * gz + iz] over the whole box (gx = 2^(L-1) npx columns,
+ * gz = 2^(L-1) npz rows; rank ip = ipx npz + ipz owns columns ipx 2^(L-1)..
+ * and rows ipz 2^(L-1)..).  Leaves are read back in each rank's Morton order,
+ * so the codes come out sorted as the reference's Mcode lists. */
+static int depth_to_codes(const mclh_input *in, int L, const unsigned char *depth, ivec *out)
+{
+    const int elx = 1 << (L - 1), max_nel = 1 << (2 * (L - 1)), np = in->npx * in->npz;
+    const size_t gz = (size_t)elx * in->npz;
+    for (int ip = 0; ip < np; ip++) {
+        int ipx = ip / in->npz, ipz = ip - ipx * in->npz;
+        int m = 0;
+        while (m < max_nel) {
+            int x, z, d, cells;
+            demorton(m, &x, &z);
+            d = depth[(size_t)(x + elx * ipx) * gz + (size_t)(z + elx * ipz)];
+            if (d > L - 1) return 1;
+            cells = 1 << (2 * (L - 1 - d));
+            if (m % cells != 0) return 1;
+            iv_push(out, mc_make(ip * max_nel + m, d));
+            m += cells;
+        }
+    }
+    return 0;
+}
+
+static void fill_depth(const mclh_input *in, int L, const int *codes, int nel, unsigned char *depth)
+{
+    const int elx = 1 << (L - 1), max_nel = 1 << (2 * (L - 1));
+    const size_t gz = (size_t)elx * in->npz;
+    for (int e = 0; e < nel; e++) {
+        int lvl = mc_lev(codes[e]), loc = mc_loc(codes[e]);
+        int ip = loc / max_nel, ipx = ip / in->npz, ipz = ip - ipx * in->npz, x, z;
+        int s = 1 << (L - 1 - lvl);
+        demorton(loc - ip * max_nel, &x, &z);
+        x += elx * ipx;
+        z += elx * ipz;
+        for (int i = 0; i < s; i++)
+            for (int j = 0; j < s; j++) depth[(size_t)(x + i) * gz + (size_t)(z + j)] = (unsigned char)lvl;
+    }
+}
+
+int mclh_build_mesh_depth(const mclh_input *in, int depth, mclh_mesh **out, char *err, int errlen)
+{
+    int L = in->max_level;
+    size_t ncell;
+    unsigned char *grid;
+    ivec top = {0};
+    *out = NULL;
+    if (check_dims(in, err, errlen)) return 1;
+    if (depth < 0 || depth > L - 1) return herr(err, errlen, "depth %d outside [0, %d]", depth, L - 1);
+    ncell = (size_t)(1 << (L - 1)) * in->npx * (size_t)(1 << (L - 1)) * in->npz;
+    grid = malloc(ncell);
+    memset(grid, depth, ncell);
+    if (depth_to_codes(in, L, grid, &top)) {
+        free(grid);
+        free(top.v);
+        return herr(err, errlen, "invalid depth grid");
+    }
+    free(grid);
+    return mesh_from_codes(in, top, out, err, errlen);
+}
+
+/* Raise the leaf block covering cell (x, z) at depth d to depth d + 1. */
+static void refine_block(unsigned char *depth, size_t gz, int L, int x, int z, int d)
+{
+    int s = 1 << (L - 1 - d), x0 = (x / s) * s, z0 = (z / s) * s;
+    for (int i = 0; i < s; i++)
+        for (int j = 0; j < s; j++) depth[(size_t)(x0 + i) * gz + (size_t)(z0 + j)] = (unsigned char)(d + 1);
+}
+
+int mclh_adapt_mesh(const mclh_input *in, const mclh_mesh *old, const double *T_old, int min_depth, int max_depth,
+                    double refine_thr, double coarsen_thr, mclh_mesh **out, double **T_new, int *nrefined,
+                    int *ncoarsened, char *err, int errlen)
+{
+    const mclh_level *f = &old->lev[old->nlev - 1];
+    const int L = old->L, elx = 1 << (L - 1), nel = f->nel, gnoz = f->gnoz;
+    const size_t gx = (size_t)elx * in->npx, gz = (size_t)elx * in->npz;
+    const double hx = old->dx[L - 1], hz = old->dz[L - 1];
+    unsigned char *od, *nd;
+    double *crit;
+    int nref = 0, ncoa = 0, changed;
+    ivec top = {0};
+    *out = NULL;
+    *T_new = NULL;
+    if (max_depth > L - 1 || min_depth < 0 || min_depth > max_depth)
+        return herr(err, errlen, "depth range [%d, %d] outside [0, %d]", min_depth, max_depth, L - 1);
+    od = malloc(gx * gz);
+    nd = malloc(gx * gz);
+    crit = malloc((size_t)(nel ? nel : 1) * sizeof(double));
+    fill_depth(in, L, f->Mcode, nel, od);
+    memcpy(nd, od, gx * gz);
+    /* criterion |grad T| h at each leaf's centre (bilinear T from its corners) */
+    for (int e = 0; e < nel; e++) {
+        int lvl = mc_lev(f->Mcode[e]), s = 1 << (L - 1 - lvl);
+        double t[4], ex = s * hx, ez = s * hz, gxv, gzv;
+        for (int a = 0; a < 4; a++) t[a] = T_old[bsearch_node(f->node, f->nno, f->ien[a * nel + e])];
+        gxv = ((t[2] + t[3]) - (t[0] + t[1])) / (2.0 * ex);
+        gzv = ((t[1] + t[2]) - (t[0] + t[3])) / (2.0 * ez);
+        crit[e] = sqrt(gxv * gxv + gzv * gzv) * (ex > ez ? ex : ez);
+    }
+    /* refinement */
+    for (int e = 0; e < nel; e++) {
+        int lvl = mc_lev(f->Mcode[e]);
+        if (crit[e] > refine_thr && lvl < max_depth) {
+            int n0 = f->ien[e], x = n0 / gnoz, z = n0 - x * gnoz;
+            refine_block(nd, gz, L, x, z, lvl);
+            nref++;
+        }
+    }
+    /* coarsening of complete sibling families that all ask for it */
+    for (int e = 0; e + 3 < nel; e++) {
+        int lvl = mc_lev(f->Mcode[e]), loc = mc_loc(f->Mcode[e]);
+        int cells = 1 << (2 * (L - 1 - lvl)), ok = 1;
+        if (lvl <= min_depth || loc % (4 * cells) != 0) continue;
+        for (int k = 0; k < 4 && ok; k++) {
+            int c = f->Mcode[e + k];
+            if (mc_lev(c) != lvl || mc_loc(c) != loc + k * cells || !(crit[e + k] < coarsen_thr)) ok = 0;
+        }
+        if (!ok) continue;
+        {
+            int n0 = f->ien[e], x = n0 / gnoz, z = n0 - x * gnoz, s = 2 << (L - 1 - lvl);
+            int x0 = (x / s) * s, z0 = (z / s) * s, unchanged = 1;
+            for (int i = 0; i < s && unchanged; i++)
+                for (int j = 0; j < s; j++)
+                    if (nd[(size_t)(x0 + i) * gz + (size_t)(z0 + j)] != lvl) { unchanged = 0; break; }
+            if (!unchanged) continue;
+            for (int i = 0; i < s; i++)
+                for (int j = 0; j < s; j++) nd[(size_t)(x0 + i) * gz + (size_t)(z0 + j)] = (unsigned char)(lvl - 1);
+            ncoa++;
+            e += 3;
+        }
+    }
+    /* 2:1 balance: a leaf next to one two or more levels deeper is refined */
+    do {
+        changed = 0;
+        for (size_t x = 0; x < gx; x++)
+            for (size_t z = 0; z < gz; z++) {
+                int d = nd[x * gz + z];
+                int dn = 0;
+                if (x > 0 && nd[(x - 1) * gz + z] > dn) dn = nd[(x - 1) * gz + z];
+                if (x + 1 < gx && nd[(x + 1) * gz + z] > dn) dn = nd[(x + 1) * gz + z];
+                if (z > 0 && nd[x * gz + z - 1] > dn) dn = nd[x * gz + z - 1];
+                if (z + 1 < gz && nd[x * gz + z + 1] > dn) dn = nd[x * gz + z + 1];
+                if (dn > d + 1) {
+                    refine_block(nd, gz, L, (int)x, (int)z, d);
+                    changed = 1;
+                }
+            }
+    } while (changed);
+    free(crit);
+    if (depth_to_codes(in, L, nd, &top)) {
+        free(od);
+        free(nd);
+        free(top.v);
+        return herr(err, errlen, "adapted depth grid is not a quadtree");
+    }
+    free(nd);
+    if (mesh_from_codes(in, top, out, err, errlen)) {
+        free(od);
+        return 1;
+    }
+    /* temperature transfer: nodes of the old mesh keep their value; new nodes
+     * take the bilinear interpolant of the old leaf that contains them */
+    {
+        const mclh_level *g = &(*out)->lev[(*out)->nlev - 1];
+        double *T = malloc((size_t)g->nno * sizeof(double));
+        for (int i = 0; i < g->nno; i++) {
+            int nid = g->node[i], k = bsearch_node(f->node, f->nno, nid);
+            if (k >= 0) {
+                T[i] = T_old[k];
+            } else {
+                int ix = nid / gnoz, iz = nid - ix * gnoz;
+                int cx = ix < (int)gx ? ix : (int)gx - 1, cz = iz < (int)gz ? iz : (int)gz - 1;
+                int d = od[(size_t)cx * gz + (size_t)cz], s = 1 << (L - 1 - d);
+                int x0 = (cx / s) * s, z0 = (cz / s) * s;
+                double xi = (double)(ix - x0) / s, eta = (double)(iz - z0) / s;
+                double t00 = T_old[bsearch_node(f->node, f->nno, z0 + x0 * gnoz)];
+                double t01 = T_old[bsearch_node(f->node, f->nno, (z0 + s) + x0 * gnoz)];
+                double t11 = T_old[bsearch_node(f->node, f->nno, (z0 + s) + (x0 + s) * gnoz)];
+                double t10 = T_old[bsearch_node(f->node, f->nno, z0 + (x0 + s) * gnoz)];
+                T[i] = (1 - xi) * (1 - eta) * t00 + (1 - xi) * eta * t01 + xi * eta * t11 + xi * (1 - eta) * t10;
+            }
+        }
+        *T_new = T;
+    }
+    free(od);
+    if (nrefined) *nrefined = nref;
+    if (ncoarsened) *ncoarsened = ncoa;
+    return 0;
+}
+
+void mclh_free_array(void *p) { free(p); }
 
 static void build_mg_maps(mclh_level *lo, mclh_level *up)
 {

@@ -105,6 +105,20 @@ void mclh_default_input(mclh_input *in);
 int mclh_set_key(mclh_input *in, const char *name, const char *value);
 int mclh_build_mesh(const mclh_input *in, mclh_mesh **out, char *err, int errlen);
 void mclh_free_mesh(mclh_mesh *m);
+/* uniform mesh of leaves at `depth` (0 .. max_level-1) with the input's levels and ranks */
+int mclh_build_mesh_depth(const mclh_input *in, int depth, mclh_mesh **out, char *err, int errlen);
+/* Quadtree adaptation of the finest level of `old` (no counterpart in the reference,
+ * whose adaptive_mesh_refine, Tree.c:95-99, is empty): leaves with |grad T| h >
+ * refine_thr are split (up to max_depth), complete sibling families with all
+ * |grad T| h < coarsen_thr are merged (down to min_depth), then leaves next to one
+ * two or more levels deeper are split until the tree is 2:1 balanced.  T_old is the
+ * finest level's nodal temperature; *T_new (malloc'd, free with mclh_free_array) gets
+ * the new finest level's: old nodes keep their value, new nodes the bilinear
+ * interpolant of the old leaf containing them. */
+int mclh_adapt_mesh(const mclh_input *in, const mclh_mesh *old, const double *T_old, int min_depth, int max_depth,
+                    double refine_thr, double coarsen_thr, mclh_mesh **out, double **T_new, int *nrefined,
+                    int *ncoarsened, char *err, int errlen);
+void mclh_free_array(void *p);
 int mclh_initial_temperature(const mclh_input *in, const mclh_mesh *m, int k, double *T, char *err, int errlen);
 void mclh_fill_view(const mclh_mesh *m, int k, mcl_mesh_view *v);
 void mclh_fill_control(const mclh_input *in, mcl_control *c);

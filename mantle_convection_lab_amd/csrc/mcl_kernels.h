@@ -51,6 +51,15 @@ struct LevelDev {
     // hanging nodes
     int nh, h_serial;
     const int *h_ni, *h_ci;          // [nh], [2][nh]
+    // Galerkin constraint u = C u_free of the mixed path: hanging equation h
+    // has hp_n[h] (1..4) free parents hp_i[k][h] with weights hp_w[k][h]
+    // (hp_n = 0 for free equations; chains of hanging parents are expanded,
+    // Dirichlet parents dropped); ch_* is the transpose (children of each
+    // free equation, CSR)
+    const int *hp_n, *hp_i;          // [neq], [4][neq]
+    const double *hp_w;              // [4][neq]
+    const int *ch_ptr, *ch_i;        // [neq + 1], [...]
+    const double *ch_w;
 };
 
 #define MCL_PHASES 8
@@ -158,6 +167,7 @@ __global__ void k_residuals(LevelDev L, RedBuf rb, const double *d, const double
 __global__ void k_collect3(RedBuf rb, int slot0, int nblocks, double *out);
 __global__ void k_heat_fp64(HeatParams H);
 __global__ void k_uzawa_mixed(MixedParams P);
+__global__ void k_uzawa_probe(MixedParams P);
 __global__ void k_to_fp32(int n, int W, const double *val, const double *diag, float *val32, float *dinv);
 __global__ void k_coarse_inverse(int n, int W, const int *col, const int *ncol, const double *val, double *inv);
 __global__ void k_inject(int nc, const int *c2f, const double *fine, double *coarse);

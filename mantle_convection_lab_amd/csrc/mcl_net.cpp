@@ -88,7 +88,8 @@ struct mcl_net {
           *hbm1 = nullptr, *hbm2 = nullptr;
     float *y = nullptr, *tgt = nullptr, *xin = nullptr, *loss = nullptr;
     long long adam_t = 0;
-    float out_scale = 1.0f;
+    float out_scale[3] = {1.0f, 1.0f, 1.0f};   // per output channel (RMS of the training targets)
+    bool scaled = false;
     // dataset of snapshots on the device: inputs [N][H][W][2], targets [N][H][W][3]
     std::vector<float *> ds_x, ds_t;
 };
@@ -461,7 +462,7 @@ float *mcl_net_in(mcl_net *n) { return n->xin; }
 float *mcl_net_out(mcl_net *n) { return n->y; }
 float *mcl_net_tgt(mcl_net *n) { return n->tgt; }
 hipStream_t mcl_net_stream(mcl_net *n) { return n->stream; }
-float mcl_net_out_scale(const mcl_net *n) { return n->out_scale; }
+float mcl_net_out_scale(const mcl_net *n, int ch) { return (ch >= 0 && ch < 3) ? n->out_scale[ch] : 0.f; }
 
 int mcl_net_dims(const mcl_net *n, int *H, int *W, int *B)
 {
@@ -479,18 +480,49 @@ int mcl_net_forward_device(mcl_net *n)
     return 0;
 }
 
-__global__ void k_sumsq(size_t n, const float *__restrict__ x, double *__restrict__ out)
+// sum of squares of each of the 3 channels of npix pixels [npix][3] (order of
+// accumulation irrelevant: it only sets the output scales)
+__global__ void k_sumsq3(size_t npix, const float *__restrict__ x, double *__restrict__ out)
 {
-    double s = 0.0;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        s += (double)x[i] * x[i];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
+    double s[3] = {0.0, 0.0, 0.0};
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < npix; i += (size_t)gridDim.x * blockDim.x)
+        for (int c = 0; c < 3; c++) s[c] += (double)x[3 * i + c] * x[3 * i + c];
+    for (int c = 0; c < 3; c++) {
+        for (int o = 32; o > 0; o >>= 1) s[c] += __shfl_xor(s[c], o, 64);
+        if ((threadIdx.x & 63) == 0) atomicAdd(out + c, s[c]);
+    }
 }
 
-__global__ void k_scale(size_t n, float a, float *__restrict__ x)
+__global__ void k_scale3(size_t npix, float a0, float a1, float a2, float *__restrict__ x)
 {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) x[i] *= a;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < npix; i += (size_t)gridDim.x * blockDim.x) {
+        x[3 * i] *= a0;
+        x[3 * i + 1] *= a1;
+        x[3 * i + 2] *= a2;
+    }
+}
+
+// Mirror one sample in x (column ix <-> W-1-ix) in place: inputs [H][W][2],
+// targets [H][W][3] with the horizontal velocity negated (the box, its
+// boundary conditions and gravity are symmetric under x -> L - x).
+__global__ void k_flip_x(int H, int W, float *__restrict__ x, float *__restrict__ t)
+{
+    const int half = W / 2;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < H * half; q += gridDim.x * blockDim.x) {
+        const int iz = q / half, ix = q - iz * half;
+        const int a = iz * W + ix, b = iz * W + (W - 1 - ix);
+        for (int c = 0; c < 2; c++) {
+            float u = x[2 * a + c], v = x[2 * b + c];
+            x[2 * a + c] = v;
+            x[2 * b + c] = u;
+        }
+        for (int c = 0; c < 3; c++) {
+            float u = t[3 * a + c], v = t[3 * b + c];
+            float sg = c == 0 ? -1.f : 1.f;
+            t[3 * a + c] = sg * v;
+            t[3 * b + c] = sg * u;
+        }
+    }
 }
 
 int mcl_net_add_snapshot(mcl_net *n, const float *x_dev, const float *t_dev)
@@ -518,20 +550,22 @@ int mcl_net_train(mcl_net *n, int epochs, float lr, unsigned long long seed, flo
         return MCL_E_STATE;
     }
     const size_t P = (size_t)n->H * n->W;
-    // output scale: RMS of all targets (set once)
-    if (n->out_scale == 1.0f) {
+    // output scales: RMS of each target channel over the dataset (set once)
+    if (!n->scaled) {
         double *acc = nullptr;
-        NCHK(hipMalloc(&acc, sizeof(double)));
-        NCHK(hipMemsetAsync(acc, 0, sizeof(double), n->stream));
+        NCHK(hipMalloc(&acc, 3 * sizeof(double)));
+        NCHK(hipMemsetAsync(acc, 0, 3 * sizeof(double), n->stream));
         for (size_t i = 0; i < N; i++)
-            hipLaunchKernelGGL(k_sumsq, dim3(gblocks(3 * P)), dim3(256), 0, n->stream, 3 * P,
-                               (const float *)n->ds_t[i], acc);
-        double s = 0.0;
-        NCHK(hipMemcpyAsync(&s, acc, sizeof(double), hipMemcpyDeviceToHost, n->stream));
+            hipLaunchKernelGGL(k_sumsq3, dim3(gblocks(P)), dim3(256), 0, n->stream, P, (const float *)n->ds_t[i], acc);
+        double sq[3] = {0.0, 0.0, 0.0};
+        NCHK(hipMemcpyAsync(sq, acc, 3 * sizeof(double), hipMemcpyDeviceToHost, n->stream));
         NCHK(hipStreamSynchronize(n->stream));
         (void)hipFree(acc);
-        double rms = std::sqrt(s / (3.0 * P * N));
-        n->out_scale = rms > 0 ? (float)rms : 1.0f;
+        for (int c = 0; c < 3; c++) {
+            double rms = std::sqrt(sq[c] / ((double)P * N));
+            n->out_scale[c] = rms > 0 ? (float)rms : 1.0f;
+        }
+        n->scaled = true;
     }
     std::mt19937_64 rng(seed);
     std::vector<size_t> order(N);
@@ -547,9 +581,12 @@ int mcl_net_train(mcl_net *n, int epochs, float lr, unsigned long long seed, flo
                                     n->stream));
                 NCHK(hipMemcpyAsync(n->tgt + k * 3 * P, n->ds_t[s], 3 * P * sizeof(float), hipMemcpyDeviceToDevice,
                                     n->stream));
+                if (rng() & 1)
+                    hipLaunchKernelGGL(k_flip_x, dim3(gblocks(P / 2)), dim3(256), 0, n->stream, n->H, n->W,
+                                       n->xin + k * 2 * P, n->tgt + k * 3 * P);
             }
-            hipLaunchKernelGGL(k_scale, dim3(gblocks(3 * P * n->B)), dim3(256), 0, n->stream, 3 * P * n->B,
-                               1.0f / n->out_scale, n->tgt);
+            hipLaunchKernelGGL(k_scale3, dim3(gblocks(P * n->B)), dim3(256), 0, n->stream, P * n->B,
+                               1.0f / n->out_scale[0], 1.0f / n->out_scale[1], 1.0f / n->out_scale[2], n->tgt);
             zero_grads(n);
             forward(n);
             backward(n);
@@ -565,11 +602,15 @@ int mcl_net_train(mcl_net *n, int epochs, float lr, unsigned long long seed, flo
     return 0;
 }
 
-double mcl_net_get_scale(const mcl_net *n) { return n ? n->out_scale : 0.0; }
-
-int mcl_net_set_scale(mcl_net *n, double scale)
+double mcl_net_get_scale(const mcl_net *n, int channel)
 {
-    if (!n || !(scale > 0)) return MCL_E_ARG;
-    n->out_scale = (float)scale;
+    return (n && channel >= 0 && channel < 3) ? n->out_scale[channel] : 0.0;
+}
+
+int mcl_net_set_scale(mcl_net *n, int channel, double scale)
+{
+    if (!n || channel < 0 || channel > 2 || !(scale > 0)) return MCL_E_ARG;
+    n->out_scale[channel] = (float)scale;
+    n->scaled = true;
     return 0;
 }

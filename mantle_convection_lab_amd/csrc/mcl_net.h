@@ -14,6 +14,6 @@ float *mcl_net_tgt(mcl_net *n);
 hipStream_t mcl_net_stream(mcl_net *n);
 int mcl_net_dims(const mcl_net *n, int *H, int *W, int *B);
 int mcl_net_forward_device(mcl_net *n);
-float mcl_net_out_scale(const mcl_net *n);
+float mcl_net_out_scale(const mcl_net *n, int channel);
 // append one snapshot (device arrays, copied) to the dataset
 int mcl_net_add_snapshot(mcl_net *n, const float *x_dev, const float *t_dev);

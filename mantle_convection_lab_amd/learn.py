@@ -75,15 +75,21 @@ class Net:
             raise MclError(f"guess failed ({rc}): {ctx.L.mcl_last_error(ctx.h).decode(errors='replace')}")
         return bool(acc.value), rn.value, rp.value
 
+    def scales(self):
+        return np.array([self.L.mcl_net_get_scale(self.h, k) for k in range(3)])
+
     def save(self, path):
         os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
-        np.savez_compressed(path, params=self.params().astype(np.float32), H=self.H, W=self.W,
-                            scale=self.L.mcl_net_get_scale(self.h))
+        # float16 storage: the convolutions run on BF16 copies of the weights anyway
+        np.savez_compressed(path, params=self.params().astype(np.float16), H=self.H, W=self.W,
+                            scale=self.scales())
 
     def load(self, path):
         with np.load(path, allow_pickle=False) as z:
-            self.set_params(z["params"])
-            self._ck(self.L.mcl_net_set_scale(self.h, float(z["scale"])))
+            self.set_params(z["params"].astype(np.float32))
+            sc = np.atleast_1d(z["scale"]).astype(np.float64)
+            for k in range(3):
+                self._ck(self.L.mcl_net_set_scale(self.h, k, float(sc[k if sc.size == 3 else 0])))
 
 
 def workload_keys(side, Ra, contrast, seed, **over):
@@ -150,7 +156,9 @@ def evaluate(weights, side, runs, steps, seed=7, log=print):
             recs = [m.step() for _ in range(steps)]
             res[label] = dict(uzawa=[x["uzawa"] for x in recs], solve_ms=[x["solve_ms"] for x in recs],
                               converged=[bool(x["converged"]) for x in recs],
-                              accepted=[bool(x.get("guess_accepted", False)) for x in recs])
+                              accepted=[bool(x.get("guess_accepted", False)) for x in recs],
+                              schur_res_net=[x.get("guess_res_net") for x in recs],
+                              schur_res_prev=[x.get("guess_res_prev") for x in recs])
             m.ctx.close()
         res.update(Ra=Ra, contrast=ctr, seed=s)
         out.append(res)
@@ -164,16 +172,16 @@ def main(argv=None):
     """Generate snapshots with the solver, train the U-Net on the GPU with the
     library's own kernels, save the weights, and evaluate on held-out runs.
 
-        python -m mantle_convection_lab_amd.learn --side 256 --runs 40 --steps 10 \
-            --epochs 60 --batch 16 --out mantle_convection_lab_amd/weights/unet_256.npz
+        python -m mantle_convection_lab_amd.learn --side 256 --runs 200 --steps 1 \
+            --epochs 150 --batch 16 --out mantle_convection_lab_amd/weights/unet_256.npz
     """
     import argparse
     import json
     ap = argparse.ArgumentParser()
     ap.add_argument("--side", type=int, default=256)
-    ap.add_argument("--runs", type=int, default=40)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--runs", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--epochs", type=int, default=150)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=1)

@@ -73,13 +73,20 @@ def _arr(ptr, n, dtype):
 class Mesh:
     """Quadtree mesh of every level (finest last), built by the C host layer."""
 
-    def __init__(self, inp):
+    def __init__(self, inp, depth=None, _ptr=None):
+        """depth=None: the input's develop_mode recipe; depth=k: uniform
+        leaves at quadtree depth k (the base mesh of an adaptive run)."""
         self.inp = inp
         L = N.host_lib()
-        self.ptr = C.POINTER(N.HostMesh)()
-        err = C.create_string_buffer(512)
-        if L.mclh_build_mesh(C.byref(inp.s), C.byref(self.ptr), err, 512):
-            raise MclError(_err(err))
+        if _ptr is not None:
+            self.ptr = _ptr
+        else:
+            self.ptr = C.POINTER(N.HostMesh)()
+            err = C.create_string_buffer(512)
+            rc = (L.mclh_build_mesh(C.byref(inp.s), C.byref(self.ptr), err, 512) if depth is None else
+                  L.mclh_build_mesh_depth(C.byref(inp.s), int(depth), C.byref(self.ptr), err, 512))
+            if rc:
+                raise MclError(_err(err))
         m = self.ptr.contents
         self.L, self.min_level, self.nlev = m.L, m.min_level, m.nlev
         self.gnox, self.gnoz = m.gnox, m.gnoz
@@ -90,6 +97,26 @@ class Mesh:
         if getattr(self, "ptr", None):
             N.host_lib().mclh_free_mesh(self.ptr)
             self.ptr = None
+
+    def adapt(self, T, min_depth, max_depth, refine_thr=0.05, coarsen_thr=0.01):
+        """New mesh refined where |grad T| h > refine_thr and coarsened where
+        it is < coarsen_thr, 2:1 balanced; returns (mesh, T on its finest
+        nodes, refined leaves, coarsened families)."""
+        L = N.host_lib()
+        T = np.ascontiguousarray(T, np.float64)
+        ptr = C.POINTER(N.HostMesh)()
+        tn = N.PD()
+        nr, nc = C.c_int(), C.c_int()
+        err = C.create_string_buffer(512)
+        if L.mclh_adapt_mesh(C.byref(self.inp.s), self.ptr, T.ctypes.data_as(N.PD), int(min_depth), int(max_depth),
+                             float(refine_thr), float(coarsen_thr), C.byref(ptr), C.byref(tn), C.byref(nr),
+                             C.byref(nc), err, 512):
+            raise MclError(_err(err))
+        new = Mesh(self.inp, _ptr=ptr)
+        nno = new.level_struct(new.nlev - 1).nno
+        Tn = np.ctypeslib.as_array(tn, shape=(nno,)).copy()
+        L.mclh_free_array(C.cast(tn, C.c_void_p))
+        return new, Tn, nr.value, nc.value
 
     def level_struct(self, k):
         return self.ptr.contents.lev[k]
@@ -247,22 +274,30 @@ class Context:
 class Model:
     """The reference's main loop (Main.c:28-69) over the device hot path."""
 
-    def __init__(self, inp, device=0, T0=None, ctrl_overrides=None, net=None):
+    def __init__(self, inp, device=0, T0=None, ctrl_overrides=None, net=None, mesh=None):
         """net: optional learned-guess network (learn.Net) on the finest
         grid; before every solve it proposes (d, p) and the candidate with the
         smaller momentum residual (network or current fields) starts the
         Uzawa iteration, which always runs to the same tolerance."""
         self.inp = inp
         self.net = net
-        self.mesh = Mesh(inp)
-        self.ctx = Context(device)
+        self.device = device
+        self.ctrl_overrides = ctrl_overrides
+        self.mesh = mesh if mesh is not None else Mesh(inp)
+        self._setup(T0)
+        self.timestep = 0
+        self.history = []
+
+    def _setup(self, T0, model_time=0.0):
+        inp = self.inp
+        self.ctx = Context(self.device)
         sizes = {}
         for k in range(self.mesh.nlev):
             l = self.mesh.level_struct(k)
             sizes[l.lev] = dict(nel=l.nel, nno=l.nno, neq=l.neq, neqT=l.neqT)
         self.ctx.upload(self.mesh.views(), sizes)
         ctrl = inp.control()
-        for k, v in (ctrl_overrides or {}).items():
+        for k, v in (self.ctrl_overrides or {}).items():
             setattr(ctrl, k, v)
         self.ctx.set_control(ctrl)
         self.finest = self.mesh.level_struct(self.mesh.nlev - 1).lev
@@ -271,9 +306,19 @@ class Model:
             lev = self.mesh.level_struct(k).lev
             T = self.mesh.initial_temperature(k) if (T0 is None or lev != self.finest) else T0
             self.ctx.set_field(lev, "T", T)
-        self.ctx.set_model_time(0.0)
-        self.timestep = 0
-        self.history = []
+        self.ctx.set_model_time(model_time)
+
+    def remesh(self, min_depth, max_depth, refine_thr=0.05, coarsen_thr=0.01):
+        """Adapt the quadtree to the current temperature (the mesh, the
+        device context and its fields are rebuilt; T is transferred, the
+        solution restarts from zero; model time is kept)."""
+        T = self.field("T")
+        t = self.ctx.model_time()
+        new, Tn, nr, nc = self.mesh.adapt(T, min_depth, max_depth, refine_thr, coarsen_thr)
+        self.ctx.close()
+        self.mesh = new
+        self._setup(Tn, model_time=t)
+        return nr, nc
 
     def step(self):
         c = self.ctx
