@@ -153,6 +153,9 @@ typedef struct mcl_control {
     int precision;             /* enum mcl_precision */
     int warm_start;            /* 1: keep d and p between steps (overrides zero_P) */
     double inner_rtol;         /* MCL_PREC_MIXED: relative tolerance of each inner solve */
+    int exact_dots;            /* FP64 path: 1 = dot products summed in the reference's sequential
+                                  order (bit-for-bit with the reference; slower), 0 = compensated
+                                  parallel sums */
 } mcl_control;
 
 typedef struct mcl_stokes_stats {

@@ -36,7 +36,7 @@ class Control(C.Structure):
                 ("check_v_only", C.c_int), ("solver", C.c_int), ("auto_accuracy", C.c_int), ("zero_P", C.c_int),
                 ("AMR", C.c_int), ("mg_cycle", C.c_int), ("mg_finest_iteration", C.c_int),
                 ("mg_coarse_iteration", C.c_int), ("schur", C.c_int), ("precision", C.c_int),
-                ("warm_start", C.c_int), ("inner_rtol", C.c_double)]
+                ("warm_start", C.c_int), ("inner_rtol", C.c_double), ("exact_dots", C.c_int)]
 
 
 class StokesStats(C.Structure):
@@ -71,7 +71,8 @@ class HostInput(C.Structure):
                 ("max_cg_cycle", C.c_int), ("old_T_file", C.c_char * 1000), ("old_T_level", C.c_int),
                 ("old_T_timestep", C.c_int), ("lg_mul", C.c_int), ("grain_damage", C.c_int),
                 ("show_convg", C.c_int), ("zero_P", C.c_int), ("schur", C.c_int), ("precision", C.c_int),
-                ("warm_start", C.c_int), ("inner_rtol", C.c_double), ("T_seed", C.c_ulonglong),
+                ("warm_start", C.c_int), ("inner_rtol", C.c_double), ("exact_dots", C.c_int),
+                ("T_seed", C.c_ulonglong),
                 ("device", C.c_int), ("write_output", C.c_int)]
 
 

@@ -1113,6 +1113,7 @@ static UzawaParams uzawa_params(mcl_ctx *c, double accuracy)
     P.cg_precondition = k.cg_precondition;
     P.check_v_only = k.check_v_only;
     P.schur = k.schur;
+    P.exact_dots = k.exact_dots;
     P.gs.nblocks = coop_blocks(c, std::max(L.neq, L.nel));
     return P;
 }
@@ -1382,6 +1383,7 @@ extern "C" int mcl_update_temperature(mcl_ctx *c, mcl_heat_stats *out)
     H.diff_dt = L.diff_dt;
     H.first_call = c->first_heat ? 1 : 0;
     H.cg_precondition = k.cg_precondition;
+    H.exact_dots = k.exact_dots;
     H.gs.nblocks = coop_blocks(c, std::max(L.neqT, L.nel));
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));

@@ -235,6 +235,41 @@ __global__ void k_assemble_K(LevelDev L, const double *__restrict__ vis, const d
     }
 }
 
+// sum_i x_i y_i in index order, rounded after every product and every
+// addition exactly as the reference's sequential loops (global_v_prod /
+// global_p_prod, Array_work.c:287-341, at one rank).  Every workgroup computes
+// it itself: the products in parallel into LDS, the additions in order by one
+// thread, so all workgroups obtain the identical value without a barrier.
+// The vectors must be complete (called after a grid barrier).
+#define MCL_SEQ_CHUNK 2048
+__device__ double seq_dot(const double *__restrict__ x, const double *__restrict__ y, int n)
+{
+    __shared__ double buf[MCL_SEQ_CHUNK];
+    __shared__ double res;
+    double s = 0.0;
+    for (int base = 0; base < n; base += MCL_SEQ_CHUNK) {
+        const int m = min(MCL_SEQ_CHUNK, n - base);
+        __syncthreads();
+        for (int t = threadIdx.x; t < m; t += blockDim.x) buf[t] = x[base + t] * y[base + t];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            // loads batched sixteen at a time, additions strictly in order
+            int t = 0;
+            for (; t + 16 <= m; t += 16) {
+                double v[16];
+#pragma unroll
+                for (int u = 0; u < 16; u++) v[u] = buf[t + u];
+#pragma unroll
+                for (int u = 0; u < 16; u++) s = s + v[u];
+            }
+            for (; t < m; t++) s = s + buf[t];
+        }
+    }
+    if (threadIdx.x == 0) res = s;
+    __syncthreads();
+    return res;
+}
+
 // Conj_Grad (Solver.c:1589-1797) / Conj_Grad_heat (Update_temperature.c:
 // 1140-1354) inside a persistent kernel.  All workgroups execute the same
 // control flow: the scalars come from reduce_collect, identical everywhere.
@@ -243,7 +278,8 @@ __global__ void k_assemble_K(LevelDev L, const double *__restrict__ vis, const d
 __device__ int pcg_persistent(const LevelDev &L, const EllMat &A, const double *__restrict__ b,
                               double *__restrict__ x, bool x_zero, double accu, int max_cycle, int precond,
                               int kind, CgVecs v, const GridSync &gs, unsigned &gen, const RedBuf &rb,
-                              dd (*lds)[MCL_BLOCK / MCL_WAVE], double *ldsm, bool &ok, long long &spmv)
+                              dd (*lds)[MCL_BLOCK / MCL_WAVE], double *ldsm, bool &ok, long long &spmv,
+                              bool exact)
 {
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsize = gridDim.x * blockDim.x;
     const int n = A.n;
@@ -265,7 +301,7 @@ __device__ int pcg_persistent(const LevelDev &L, const EllMat &A, const double *
     {
         double o[1];
         reduce_collect<1>(rb, 0, gridDim.x, o, lds);
-        delta_new = o[0];
+        delta_new = exact ? seq_dot(v.r, v.d, n) : o[0];
     }
     const double delta0 = delta_new;
     int cycle = 0;
@@ -286,7 +322,7 @@ __device__ int pcg_persistent(const LevelDev &L, const EllMat &A, const double *
         {
             double o[1];
             reduce_collect<1>(rb, 1, gridDim.x, o, lds);
-            t1 = o[0];
+            t1 = exact ? seq_dot(v.d, v.q, n) : o[0];
         }
         const double alpha = delta_new / t1;
         for (int i = gtid; i < n; i += gsize) x[i] += alpha * v.d[i];
@@ -321,7 +357,7 @@ __device__ int pcg_persistent(const LevelDev &L, const EllMat &A, const double *
         {
             double o[1];
             reduce_collect<1>(rb, 2, gridDim.x, o, lds);
-            delta_new = o[0];
+            delta_new = exact ? seq_dot(v.r, v.s, n) : o[0];
             double m = 0.0;
             for (int bb = threadIdx.x; bb < (int)gridDim.x; bb += MCL_BLOCK) m = fmax(m, rb.pmax[bb]);
             res = block_max(m, ldsm);
@@ -371,7 +407,7 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_uzawa_fp64(UzawaParams P)
     spmv++;
     ok = ok && grid_sync(P.gs, gen);
     inner += pcg_persistent(L, K, P.Ahv, P.u1, false, P.accuracy, P.max_cg_cycle, P.cg_precondition, 0, cv,
-                            P.gs, gen, P.rb, lds, ldsm, ok, spmv);
+                            P.gs, gen, P.rb, lds, ldsm, ok, spmv, P.exact_dots != 0);
     for (int i = gtid; i < neq; i += gsize) P.d[i] += P.u1[i];
     ok = ok && grid_sync(P.gs, gen);
     double *r0 = P.r0, *r1 = P.r1, *r2 = P.r2, *z0 = P.z0, *z1 = P.z1, *s1 = P.s1, *s2 = P.s2;
@@ -396,6 +432,10 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_uzawa_fp64(UzawaParams P)
         {
             double o[2];
             reduce_collect<2>(P.rb, 4, gridDim.x, o, lds);
+            if (P.exact_dots) {
+                o[0] = seq_dot(r1, z1, nel);
+                if (cycle > 0) o[1] = seq_dot(r0, z0, nel);
+            }
             r1z1 = o[0];
             if (cycle > 0) delta = r1z1 / o[1];
         }
@@ -409,7 +449,7 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_uzawa_fp64(UzawaParams P)
         }
         ok = ok && grid_sync(P.gs, gen);
         inner += pcg_persistent(L, K, P.Ahv, P.u1, true, P.accuracy, P.max_cg_cycle, P.cg_precondition, 0, cv,
-                                P.gs, gen, P.rb, lds, ldsm, ok, spmv);
+                                P.gs, gen, P.rb, lds, ldsm, ok, spmv, P.exact_dots != 0);
         // Ah = G^T u1 ; alpha = r1.z1 / s2.Ah
         {
             dd acc[1] = {dd_zero()};
@@ -425,6 +465,7 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_uzawa_fp64(UzawaParams P)
         {
             double o[1];
             reduce_collect<1>(P.rb, 6, gridDim.x, o, lds);
+            if (P.exact_dots) o[0] = seq_dot(s2, P.Ahp, nel);
             alpha = r1z1 / o[0];
         }
         {
@@ -449,6 +490,12 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_uzawa_fp64(UzawaParams P)
         {
             double o[4];
             reduce_collect<4>(P.rb, 8, gridDim.x, o, lds);
+            if (P.exact_dots) {
+                o[0] = seq_dot(s2, s2, nel);
+                o[1] = seq_dot(P.p, P.p, nel);
+                o[2] = seq_dot(P.u1, P.u1, neq);
+                o[3] = seq_dot(P.d, P.d, neq);
+            }
             dp = alpha * sqrt(o[0] / (1.0e-32 + o[1]));
             dv = alpha * sqrt(o[2] / (1.0e-32 + o[3]));
         }
@@ -707,7 +754,7 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_heat_fp64(HeatParams H)
         }
         ok = ok && grid_sync(H.gs, gen);
         init_cycles = pcg_persistent(L, Ch, H.B, H.hdd, true, 1e-3, 400, H.cg_precondition, 1, cv, H.gs, gen,
-                                     H.rb, lds, ldsm, ok, spmv);
+                                     H.rb, lds, ldsm, ok, spmv, H.exact_dots != 0);
     }
     const double td = H.theta * dt;
     for (int i = gtid; i < n; i += gsize) {
@@ -718,7 +765,7 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_heat_fp64(HeatParams H)
     }
     ok = ok && grid_sync(H.gs, gen);
     int cycles = pcg_persistent(L, Ah, H.B, H.hd, false, 1e-3, 1000, H.cg_precondition, 1, cv, H.gs, gen, H.rb, lds,
-                                ldsm, ok, spmv);
+                                ldsm, ok, spmv, H.exact_dots != 0);
     for (int i = gtid; i < L.nno; i += gsize) {
         int k = L.heat_id[i];
         H.T[i] = k >= 0 ? H.hd[k] : L.T_bc_val[i];

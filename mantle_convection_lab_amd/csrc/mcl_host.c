@@ -119,6 +119,7 @@ int mclh_set_key(mclh_input *in, const char *k, const char *v)
     I("grain_damage", grain_damage) I("show_convg", show_convg) I("zero_P", zero_P)
     /* keys of this build */
     I("schur_mode", schur) I("precision", precision) I("warm_start", warm_start) D("inner_rtol", inner_rtol)
+    I("exact_dots", exact_dots)
     I("device", device) I("write_output", write_output)
     if (!strcmp(k, "T_seed")) { in->T_seed = strtoull(v, NULL, 10); return 0; }
 #undef I
@@ -198,6 +199,7 @@ void mclh_fill_control(const mclh_input *in, mcl_control *c)
     c->precision = in->precision;
     c->warm_start = in->warm_start;
     c->inner_rtol = in->inner_rtol;
+    c->exact_dots = in->exact_dots;
 }
 
 /* ---------------------------------------------------------------- quadtree */
@@ -945,6 +947,21 @@ int mclh_adapt_mesh(const mclh_input *in, const mclh_mesh *old, const double *T_
 }
 
 void mclh_free_array(void *p) { free(p); }
+
+/* sizes of the structs shared with bindings, for layout checks */
+long mclh_sizeof(int which)
+{
+    switch (which) {
+    case 0: return (long)sizeof(mclh_input);
+    case 1: return (long)sizeof(mcl_control);
+    case 2: return (long)sizeof(mcl_mesh_view);
+    case 3: return (long)sizeof(mcl_stokes_stats);
+    case 4: return (long)sizeof(mcl_heat_stats);
+    case 5: return (long)sizeof(mclh_level);
+    case 6: return (long)sizeof(mclh_mesh);
+    default: return -1;
+    }
+}
 
 static void build_mg_maps(mclh_level *lo, mclh_level *up)
 {

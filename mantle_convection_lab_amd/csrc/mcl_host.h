@@ -58,6 +58,7 @@ typedef struct mclh_input {
     int precision;               /* 0 fp64, 1 mixed */
     int warm_start;
     double inner_rtol;
+    int exact_dots;              /* FP64 path: sequential dot products (bit-for-bit) */
     unsigned long long T_seed;   /* T_ini = 4: seeded perturbation */
     int device;
     int write_output;            /* 0: skip Output.c files */
@@ -119,6 +120,9 @@ int mclh_adapt_mesh(const mclh_input *in, const mclh_mesh *old, const double *T_
                     double refine_thr, double coarsen_thr, mclh_mesh **out, double **T_new, int *nrefined,
                     int *ncoarsened, char *err, int errlen);
 void mclh_free_array(void *p);
+/* sizeof of mclh_input, mcl_control, mcl_mesh_view, mcl_stokes_stats, mcl_heat_stats,
+ * mclh_level, mclh_mesh (which = 0..6), for checking binding layouts */
+long mclh_sizeof(int which);
 int mclh_initial_temperature(const mclh_input *in, const mclh_mesh *m, int k, double *T, char *err, int errlen);
 void mclh_fill_view(const mclh_mesh *m, int k, mcl_mesh_view *v);
 void mclh_fill_control(const mclh_input *in, mcl_control *c);

@@ -81,6 +81,7 @@ struct UzawaParams {
     double *cg_r, *cg_s, *cg_d, *cg_q;
     double accuracy, citcom_accu;
     int max_citcom_cycle, max_cg_cycle, cg_precondition, check_v_only, schur;
+    int exact_dots;          // dot products in the reference's sequential order
 };
 
 struct HeatParams {
@@ -94,6 +95,7 @@ struct HeatParams {
     double *model_time;
     double accuracy, theta, Q, Llast0, Llast1, diff_dt;
     int first_call, cg_precondition;
+    int exact_dots;
 };
 
 // One multigrid level of the FP32 inner solver (mixed precision).  Level 0 is

@@ -38,3 +38,16 @@ def test_null_arguments_rejected():
     assert lib.mcl_create(0, None) != 0
     assert lib.mcl_mesh_upload(None, None, 0) != 0
     assert lib.mcl_solve_velocity_pressure(None, None) != 0
+
+
+def test_binding_struct_layouts_match_c():
+    """Every ctypes mirror of a C struct has the C struct's size (a field
+    added on one side only would shift the others and corrupt memory)."""
+    import ctypes as C
+    from mantle_convection_lab_amd import _native as N
+    L = N.host_lib()
+    L.mclh_sizeof.argtypes = [C.c_int]
+    L.mclh_sizeof.restype = C.c_long
+    for which, st in enumerate([N.HostInput, N.Control, N.MeshView, N.StokesStats, N.HeatStats, N.HostLevel,
+                                N.HostMesh]):
+        assert C.sizeof(st) == L.mclh_sizeof(which), st.__name__
