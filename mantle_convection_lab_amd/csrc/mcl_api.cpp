@@ -80,7 +80,7 @@ struct mcl_ctx {
     double *model_time = nullptr;    // device scalar
     double *resid = nullptr;         // device [3]
     // mixed precision: multigrid level table, smoother bounds, coarse inverse, FP32 CG vectors
-    MgLevelDev *mg_dev = nullptr;
+    MgLevelDev *mg_dev = nullptr;      // level descriptors of the mixed solve
     unsigned *lam_bits = nullptr;
     double *cinv = nullptr;
     double *rr = nullptr;
@@ -404,6 +404,9 @@ static int build_level(mcl_ctx *c, const mcl_mesh_view &v, Level &Lv)
     if (v.interp_n && v.interp_eq) {
         Lv.interp_n.assign(v.interp_n, v.interp_n + neq);
         Lv.interp_eq.assign(v.interp_eq, v.interp_eq + 4 * (size_t)neq);
+        for (int u = 0; u < neq; u++)
+            if (Lv.interp_n[u] < 0 || Lv.interp_n[u] > 4)
+                return fail(c, MCL_E_ARG, "interp_n[%d] = %d outside 0..4", u, Lv.interp_n[u]);
     }
     // buoyancy row-average contribution lists, in the reference's order
     // (get_node_horiz_avg, Parallel.c:354-437)
@@ -671,16 +674,25 @@ static int build_multigrid(mcl_ctx *c)
     }
     // coarse levels run per workgroup from LDS: level 0 (the dense solve) and
     // the levels above it while each has at most block_rows_threshold() rows,
-    // their rows sum to at most MCL_MG_LDS_ROWS, and the level is not the finest
+    // their rows sum to at most MCL_MG_LDS_ROWS, their ELL entries to at most
+    // MCL_MG_LDS_NNZ, and the level is not the finest
     {
-        int off = 0;
+        int off = 0, nnz = 0;
         for (int k = 0; k < nlev - 1; k++) {
-            const int n = c->levels[k].neq;
-            if (k > 0 && (n > block_rows_threshold() || off / 3 + n > MCL_MG_LDS_ROWS)) break;
-            if (off / 3 + n > MCL_MG_LDS_ROWS) break;
+            Level &Lv = c->levels[k];
+            const int n = Lv.neq, m = Lv.WK * n;
+            if (k > 0 && n > block_rows_threshold()) break;
+            if (off / 3 + n > MCL_MG_LDS_ROWS || nnz + m > MCL_MG_LDS_NNZ || n > 65535) break;
+            std::vector<unsigned short> col16(std::max(m, 1), 0);
+            for (int e = 0; e < m; e++) col16[e] = (unsigned short)Lv.Kcol_h[e];
+            const unsigned short *cp;
+            if (dev_upload(c, Lv.owned, &cp, col16)) return MCL_E_DEVICE;
+            mg[k].col16 = cp;
             mg[k].block = 1;
             mg[k].lds_off = off;
+            mg[k].moff = nnz;
             off += 3 * n;
+            nnz += m;
             c->mg_lb = k;
         }
     }
@@ -740,8 +752,11 @@ static int build_multigrid(mcl_ctx *c)
             c->owned.erase(std::remove(c->owned.begin(), c->owned.end(), p), c->owned.end());
         }
     }
-    c->mg_dev = nullptr;
     int rc = 0;
+    // deeper hierarchies than the kernel's LDS copy holds use the Jacobi
+    // preconditioner (a larger mg_min_level shortens the hierarchy)
+    if (nlev > MCL_MG_MAX_LEVELS) maps = false;
+    c->mg_dev = nullptr;
     if (maps) {
         rc |= dev_alloc(c, c->owned, &c->mg_dev, nlev);
         if (!rc) rc |= hipMemcpy(c->mg_dev, mg.data(), nlev * sizeof(MgLevelDev), hipMemcpyHostToDevice) != hipSuccess;
@@ -1052,19 +1067,21 @@ extern "C" int mcl_velocity_pressure_KGF_matrix(mcl_ctx *c)
     return 0;
 }
 
-// Workgroups of a persistent solve: one per MCL_ROWS_PER_THREAD * MCL_BLOCK
-// rows (a grid barrier costs more with more workgroups; a few rows per thread
-// keep enough loads in flight), at most what fits co-resident, and at most
-// MCL_COOP_BLOCKS if that is set.
-#define MCL_ROWS_PER_THREAD 4
-static unsigned coop_blocks(mcl_ctx *c, long long n)
+// Workgroups of a persistent solve: one per rows_per_thread * MCL_BLOCK rows,
+// at most what fits co-resident; the variable MCL_COOP_BLOCKS, if set,
+// replaces the count (still capped at co-residency).  The FP64 solvers take
+// two rows per thread (their phases are grid barriers, whose cost grows with
+// the workgroup count); the mixed solve takes one (its V-cycle waits are
+// mostly point-to-point, and more workgroups keep more loads in flight).
+#define MCL_FP64_ROWS_PER_THREAD 2
+#define MCL_MIXED_ROWS_PER_THREAD 1
+static unsigned coop_blocks(mcl_ctx *c, long long n, int rows_per_thread)
 {
-    const long long per = (long long)MCL_ROWS_PER_THREAD * MCL_BLOCK;
+    const long long per = (long long)rows_per_thread * MCL_BLOCK;
     long long b = (n + per - 1) / per;
-    b = std::max<long long>(1, std::min<long long>(b, c->max_coop_blocks));
     const char *e = getenv("MCL_COOP_BLOCKS");
-    if (e && atoi(e) > 0) b = std::min<long long>(b, atoi(e));
-    return (unsigned)b;
+    if (e && atoi(e) > 0) b = atoi(e);
+    return (unsigned)std::max<long long>(1, std::min<long long>(b, c->max_coop_blocks));
 }
 
 static int reset_sync_if_aborted(mcl_ctx *c, const DevStats &st)
@@ -1114,7 +1131,7 @@ static UzawaParams uzawa_params(mcl_ctx *c, double accuracy)
     P.check_v_only = k.check_v_only;
     P.schur = k.schur;
     P.exact_dots = k.exact_dots;
-    P.gs.nblocks = coop_blocks(c, std::max(L.neq, L.nel));
+    P.gs.nblocks = coop_blocks(c, std::max(L.neq, L.nel), MCL_FP64_ROWS_PER_THREAD);
     return P;
 }
 
@@ -1217,6 +1234,7 @@ static MixedParams mixed_params(mcl_ctx *c, const UzawaParams &P)
     const mcl_control &k = c->ctl;
     MixedParams M{};
     M.U = P;
+    M.U.gs.nblocks = coop_blocks(c, std::max(P.L.neq, P.L.nel), MCL_MIXED_ROWS_PER_THREAD);
     M.mg = c->mg_dev;
     M.lam = (const float *)c->lam_bits;
     M.nmg = c->have_mg ? c->nlev : 1;
@@ -1227,9 +1245,10 @@ static MixedParams mixed_params(mcl_ctx *c, const UzawaParams &P)
     M.max_inner = 1000;
     M.probe = 0;
     M.use_nbr = 0;
-    if (c->have_mg && (int)P.gs.nblocks != c->nbr_nb && build_neighbours(c, (int)P.gs.nblocks) == 0) {
+    const int nb = (int)M.U.gs.nblocks;
+    if (c->have_mg && nb != c->nbr_nb && build_neighbours(c, nb) == 0) {
     }
-    if (c->have_mg && c->nbr_nb == (int)P.gs.nblocks) {
+    if (c->have_mg && c->nbr_nb == nb) {
         M.nbr_ptr = c->nbr_ptr;
         M.nbr_idx = c->nbr_idx;
         M.use_nbr = c->nbr_ok;
@@ -1282,7 +1301,7 @@ extern "C" int mcl_solve_velocity_pressure(mcl_ctx *c, mcl_stokes_stats *out)
         MixedParams M = mixed_params(c, P);
         void *margs[] = {&M};
         HIPCHK(c, hipEventRecord(e0, c->stream));
-        HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_uzawa_mixed, dim3(P.gs.nblocks), dim3(MCL_BLOCK), margs,
+        HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_uzawa_mixed, dim3(M.U.gs.nblocks), dim3(MCL_BLOCK), margs,
                                              0, c->stream));
     } else {
         void *args[] = {&P};
@@ -1384,7 +1403,7 @@ extern "C" int mcl_update_temperature(mcl_ctx *c, mcl_heat_stats *out)
     H.first_call = c->first_heat ? 1 : 0;
     H.cg_precondition = k.cg_precondition;
     H.exact_dots = k.exact_dots;
-    H.gs.nblocks = coop_blocks(c, std::max(L.neqT, L.nel));
+    H.gs.nblocks = coop_blocks(c, std::max(L.neqT, L.nel), MCL_FP64_ROWS_PER_THREAD);
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
@@ -1620,7 +1639,7 @@ static int uzawa_probe(mcl_ctx *c, double *d, double *p, double *r1)
         M.inner_rtol = e && atof(e) > 0 ? atof(e) : 1e-3;
     }
     void *margs[] = {&M};
-    HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_uzawa_probe, dim3(P.gs.nblocks), dim3(MCL_BLOCK), margs, 0,
+    HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_uzawa_probe, dim3(M.U.gs.nblocks), dim3(MCL_BLOCK), margs, 0,
                                          c->stream));
     DevStats st;
     HIPCHK(c, hipMemcpyAsync(&st, c->stats, sizeof st, hipMemcpyDeviceToHost, c->stream));

@@ -7,7 +7,7 @@
 #include <stdint.h>
 
 #ifndef MCL_BLOCK
-#define MCL_BLOCK 256   // threads per workgroup of every kernel
+#define MCL_BLOCK 512   // threads per workgroup of every kernel (8 waves: one CU each)
 #endif
 #define MCL_WAVE 64
 #define MCL_MAX_LEVELS 20
@@ -113,6 +113,32 @@ __device__ __forceinline__ bool nbr_sync(const GridSync &gs, unsigned &my_gen, i
         spin_until(gs.arrive + nbr, target, gs.abort);
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
+    if (threadIdx.x == 0) s_abort = __hip_atomic_load(gs.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    return s_abort == 0;
+}
+
+// Neighbour wait for phases whose cross-workgroup vectors are accessed only
+// with agent-scope atomic loads and stores (coherent at the device level: a
+// store writes through the XCD's L2, a load never returns a stale L2 line).
+// Every wave first waits for all its memory operations to complete
+// (s_waitcnt 0: its stores are visible device-wide and its loads have
+// returned), so the arrival flag is ordered after them without an L2
+// write-back; a waiter issues its next loads only after it has seen the flag,
+// and they are coherent, so no L2 invalidate is needed either.  The L2 thus
+// keeps the read-only matrices across phases.  Plain loads or stores of those
+// vectors must stay on the far side of a fenced grid_sync.
+__device__ __forceinline__ bool nbr_sync_coherent(const GridSync &gs, unsigned &my_gen, int nbr)
+{
+    __shared__ unsigned s_abort;
+    const unsigned target = my_gen + 1;
+    my_gen = target;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        __hip_atomic_store(gs.arrive + blockIdx.x, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (nbr >= 0) spin_until(gs.arrive + nbr, target, gs.abort);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
     if (threadIdx.x == 0) s_abort = __hip_atomic_load(gs.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     return s_abort == 0;

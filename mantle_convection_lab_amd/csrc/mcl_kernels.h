@@ -62,7 +62,7 @@ struct LevelDev {
     const double *ch_w;
 };
 
-#define MCL_PHASES 8
+#define MCL_PHASES 12
 struct DevStats {
     int cycles, init_cycles, aborted, nan_flag;
     double dv, dp, dt;
@@ -104,12 +104,21 @@ struct HeatParams {
 // workgroup (LDS-speed barriers); the others are processed by the whole grid.
 // Rows (summed over levels) of the coarse multigrid levels that every
 // workgroup of the mixed solve keeps in LDS (three float vectors per row).
+#ifndef MCL_MG_LDS_ROWS
 #define MCL_MG_LDS_ROWS 1536
+#endif
+// Matrix entries (ELL slots summed over levels) of those levels, also kept in
+// LDS (FP32 values and 16-bit columns, 6 bytes an entry).
+#ifndef MCL_MG_LDS_NNZ
+#define MCL_MG_LDS_NNZ 16384
+#endif
 
 struct MgLevelDev {
     int n, WK;
     int block;               // coarse level run per workgroup from LDS
     int lds_off;             // offset of its vectors in the LDS block (floats)
+    int moff;                // offset of its matrix in the LDS matrix block (entries)
+    const unsigned short *col16;  // [WK][n] col as 16-bit indices (LDS levels)
     const int *col;          // [WK][n] (the level's FP64 ELL pattern)
     const int *ncol;         // [n]
     float *val;              // [WK][n] FP32 copy of the level's K
@@ -126,6 +135,10 @@ struct MgLevelDev {
     const float *rval;       // [rw][n_coarse]
     float *x, *b, *r, *t;    // level vectors
 };
+
+// Multigrid levels the mixed solve can hold (every workgroup keeps a copy of
+// their descriptors in LDS).
+#define MCL_MG_MAX_LEVELS 14
 
 struct MixedParams {
     UzawaParams U;

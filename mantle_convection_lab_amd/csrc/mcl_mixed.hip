@@ -17,8 +17,9 @@
 // workgroup 0 charges the 100 MHz wall clock since its previous stamp to a
 // phase; grid barriers keep every workgroup in step, so workgroup 0's split
 // is the grid's.  Phases: 0 d'/q pass, 1 x/r update (+ start), 2 V-cycle down
-// (grid levels), 3 workgroup-0 levels, 4 V-cycle up, 5 last sweep + r.z,
-// 6 Uzawa outer work.
+// (grid levels), 7 restriction into the LDS levels, 8 LDS levels down,
+// 9 coarsest solve, 3 LDS levels up, 4 V-cycle up (grid levels), 5 last
+// sweep + r.z, 6 Uzawa outer work.
 #if MCL_PHASE_TIMERS
 __shared__ long long mcl_ph[MCL_PHASES + 1];
 #define MCL_PHASE(k)                                                           \
@@ -128,7 +129,125 @@ __device__ __forceinline__ Ex own_rows(int n)
 // all rows of a level, by one workgroup (the LDS levels)
 __device__ __forceinline__ Ex wg_rows(int n) { return Ex{(int)threadIdx.x, n, (int)blockDim.x}; }
 
-__device__ __forceinline__ float ell32(const MgLevelDev &A, int i, const float *__restrict__ x)
+// Address spaces of the V-cycle's data: grid levels live in global memory
+// (AS 1), the coarse levels' vectors and matrices in LDS (AS 3).  Typed
+// pointers make the loads global_load / ds_read instead of flat accesses that
+// wait on both memory counters.  The level descriptors are read from the
+// workgroup's LDS copy; the pointers they hold are cast to AS 1.
+#define MCL_G __attribute__((address_space(1)))
+#define MCL_L __attribute__((address_space(3)))
+template <typename T>
+__device__ __forceinline__ MCL_G T *gp(T *p)
+{
+    return (MCL_G T *)p;
+}
+
+// Vector accesses of the V-cycle.  CO = true for vectors of the grid levels
+// between neighbour waits: agent-scope atomic loads and stores (coherent
+// across the XCDs' L2 caches, see nbr_sync_coherent); CO = false for LDS
+// vectors and plain global accesses.
+#ifndef MCL_VCYCLE_COHERENT
+#define MCL_VCYCLE_COHERENT 1
+#endif
+constexpr bool kCO = MCL_VCYCLE_COHERENT != 0;
+
+template <bool CO, typename Ptr>
+__device__ __forceinline__ float vld(Ptr p)
+{
+    if constexpr (CO) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return *p;
+}
+
+template <bool CO, typename Ptr>
+__device__ __forceinline__ void vst(Ptr p, float v)
+{
+    if constexpr (CO) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+}
+
+// A grid level as the level operations see it.
+struct GLevel {
+    int n, WK, rw;
+    const MCL_G int *col;
+    const MCL_G float *val, *dinv;
+    const MCL_G int *pn, *pcol, *rcol;
+    const MCL_G float *rval;
+    MCL_G float *x, *b, *r, *t;
+};
+
+// A coarse level as seen by one workgroup: vectors x, t, b (3 n floats at
+// lds_off), 1/diag(K) (n floats at lds_off / 3) and the FP32 K with 16-bit
+// columns (WK n entries at moff) in LDS, copied from HBM once per launch; the
+// transfer maps stay in HBM.
+struct LdsLevel {
+    int n, WK, rw;
+    const MCL_L unsigned short *col;
+    const MCL_L float *val, *dinv;
+    const MCL_G int *pn, *pcol, *rcol;
+    const MCL_G float *rval;
+    MCL_L float *x, *b, *r, *t;
+};
+
+// the workgroup's LDS blocks: level descriptors, Gershgorin bounds (both read
+// in every phase, so kept out of the grid barriers' cache invalidations), and
+// the coarse levels' vectors and matrices
+struct MgLds {
+    const MCL_L MgLevelDev *lev;
+    const MCL_L float *lam;
+    MCL_L float *vec, *dinv, *val;
+    MCL_L unsigned short *col;
+};
+
+template <typename LV>
+__device__ __forceinline__ auto sweep_buf(const LV &A, int k)
+{
+    return (k & 1) ? A.x : A.t;
+}
+
+__device__ __forceinline__ GLevel glevel(const MCL_L MgLevelDev *G)
+{
+    GLevel A;
+    A.n = G->n;
+    A.WK = G->WK;
+    A.rw = G->rw;
+    A.col = gp(G->col);
+    A.val = gp(G->val);
+    A.dinv = gp(G->dinv);
+    A.pn = gp(G->pn);
+    A.pcol = gp(G->pcol);
+    A.rcol = gp(G->rcol);
+    A.rval = gp(G->rval);
+    A.x = gp(G->x);
+    A.b = gp(G->b);
+    A.r = gp(G->r);
+    A.t = gp(G->t);
+    return A;
+}
+
+// the residual reuses the sweep buffer that is free after the pre-sweeps
+__device__ __forceinline__ LdsLevel lds_level(const MgLds &S, int l, int pre)
+{
+    const MCL_L MgLevelDev *G = S.lev + l;
+    LdsLevel A;
+    A.n = G->n;
+    A.WK = G->WK;
+    A.rw = G->rw;
+    A.col = S.col + G->moff;
+    A.val = S.val + G->moff;
+    A.dinv = S.dinv + G->lds_off / 3;
+    A.pn = gp(G->pn);
+    A.pcol = gp(G->pcol);
+    A.rcol = gp(G->rcol);
+    A.rval = gp(G->rval);
+    A.x = S.vec + G->lds_off;
+    A.t = A.x + A.n;
+    A.b = A.t + A.n;
+    A.r = sweep_buf(A, pre - 1);
+    return A;
+}
+
+template <bool CO, typename LV, typename XP>
+__device__ __forceinline__ float ell32(const LV &A, int i, XP x)
 {
     // Rows of width 18 (the bilinear-quad velocity stencil) issue all their
     // loads at once, other widths nine entries per round trip; the sum keeps
@@ -143,7 +262,7 @@ __device__ __forceinline__ float ell32(const MgLevelDev &A, int i, const float *
             v[u] = A.val[u * A.n + i];
         }
 #pragma unroll
-        for (int u = 0; u < 18; u++) g[u] = x[c[u]];
+        for (int u = 0; u < 18; u++) g[u] = vld<CO>(x + c[u]);
 #pragma unroll
         for (int u = 0; u < 18; u++) s += v[u] * g[u];
         return s;
@@ -151,64 +270,87 @@ __device__ __forceinline__ float ell32(const MgLevelDev &A, int i, const float *
     int k = 0;
     for (; k + 9 <= A.WK; k += 9) {
         int c[9];
-        float v[9];
+        float v[9], g[9];
 #pragma unroll
         for (int u = 0; u < 9; u++) {
             c[u] = A.col[(k + u) * A.n + i];
             v[u] = A.val[(k + u) * A.n + i];
         }
 #pragma unroll
-        for (int u = 0; u < 9; u++) s += v[u] * x[c[u]];
+        for (int u = 0; u < 9; u++) g[u] = vld<CO>(x + c[u]);
+#pragma unroll
+        for (int u = 0; u < 9; u++) s += v[u] * g[u];
     }
-    for (; k < A.WK; k++) s += A.val[k * A.n + i] * x[A.col[k * A.n + i]];
+    for (; k < A.WK; k++) s += A.val[k * A.n + i] * vld<CO>(x + A.col[k * A.n + i]);
     return s;
 }
 
-__device__ __forceinline__ float *sweep_buf(const MgLevelDev &A, int k) { return (k & 1) ? A.x : A.t; }
-
 // sweep k >= 2: x_k = x_{k-1} + w D^-1 (b - K x_{k-1})
-__device__ __forceinline__ void jacobi(const MgLevelDev &A, int k, float w, Ex ex)
+template <bool CO, typename LV>
+__device__ __forceinline__ void jacobi(const LV &A, int k, float w, Ex ex)
 {
-    const float *__restrict__ in = sweep_buf(A, k - 1);
-    float *__restrict__ out = sweep_buf(A, k);
-    for (int i = ex.b; i < ex.e; i += ex.s) out[i] = in[i] + w * A.dinv[i] * (A.b[i] - ell32(A, i, in));
+    const auto in = sweep_buf(A, k - 1);
+    const auto out = sweep_buf(A, k);
+    for (int i = ex.b; i < ex.e; i += ex.s)
+        vst<CO>(out + i, vld<CO>(in + i) + w * A.dinv[i] * (vld<CO>(A.b + i) - ell32<CO>(A, i, in)));
 }
 
 // r = b - K x on a level
-__device__ __forceinline__ void residual(const MgLevelDev &A, const float *__restrict__ x, Ex ex)
+template <bool CO, typename LV, typename XP>
+__device__ __forceinline__ void residual(const LV &A, XP x, Ex ex)
 {
-    for (int i = ex.b; i < ex.e; i += ex.s) A.r[i] = A.b[i] - ell32(A, i, x);
+    for (int i = ex.b; i < ex.e; i += ex.s) vst<CO>(A.r + i, vld<CO>(A.b + i) - ell32<CO>(A, i, x));
 }
 
 // Coarse right-hand side b_c = sum_k R_ck r_u (R = P^T, arrays on the coarse
 // level), fused with the coarse level's first sweep x_c = w_c D^-1 b_c unless
-// the coarse level is solved directly.
-__device__ __forceinline__ void restrict_to(const MgLevelDev &C, const MgLevelDev &F, bool sweep, float wc, Ex ex)
+// the coarse level is solved directly.  CF: access mode of the fine level's
+// vectors, CC: of the coarse level's.  Nine entries per round trip (all map
+// loads, then all gathers); the sum keeps the slot order.
+template <bool CF, bool CC, typename LC, typename LF>
+__device__ __forceinline__ void restrict_to(const LC &C, const LF &F, bool sweep, float wc, Ex ex)
 {
     for (int c = ex.b; c < ex.e; c += ex.s) {
         float s = 0.f;
-        for (int k = 0; k < C.rw; k++) {
-            int u = C.rcol[k * C.n + c];
-            if (u >= 0) s += C.rval[k * C.n + c] * F.r[u];
+        for (int k0 = 0; k0 < C.rw; k0 += 9) {
+            int u[9];
+            float v[9], g[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                const bool in = k0 + k < C.rw;
+                u[k] = in ? C.rcol[(k0 + k) * C.n + c] : -1;
+                v[k] = in ? C.rval[(k0 + k) * C.n + c] : 0.f;
+            }
+#pragma unroll
+            for (int k = 0; k < 9; k++) g[k] = u[k] >= 0 ? vld<CF>(F.r + u[k]) : 0.f;
+#pragma unroll
+            for (int k = 0; k < 9; k++)
+                if (u[k] >= 0) s += v[k] * g[k];
         }
-        C.b[c] = s;
-        if (sweep) C.x[c] = wc * C.dinv[c] * s;
+        vst<CC>(C.b + c, s);
+        if (sweep) vst<CC>(C.x + c, wc * C.dinv[c] * s);
     }
 }
 
-// x_f += P x_c: fine row u takes the mean of its pn[u] coarse rows
-__device__ __forceinline__ void prolong_add(const MgLevelDev &F, float *__restrict__ xf, const MgLevelDev &C,
-                                            const float *__restrict__ xc, Ex ex)
+// x_f += P x_c: fine row u takes the mean of its pn[u] coarse rows (pcol has
+// four slots, -1 past pn[u] and for fixed coarse rows; all loads issued at once)
+template <bool CF, bool CC, typename LF, typename XF, typename XC>
+__device__ __forceinline__ void prolong_add(const LF &F, XF xf, XC xc, Ex ex)
 {
     for (int u = ex.b; u < ex.e; u += ex.s) {
-        int m = F.pn[u];
+        const int m = F.pn[u];
+        int l[4];
+        float g[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) l[k] = F.pcol[k * F.n + u];
+#pragma unroll
+        for (int k = 0; k < 4; k++) g[k] = (k < m && l[k] >= 0) ? vld<CC>(xc + l[k]) : 0.f;
         if (m == 0) continue;
         float s = 0.f;
-        for (int k = 0; k < m; k++) {
-            int l = F.pcol[k * F.n + u];
-            if (l >= 0) s += xc[l];
-        }
-        xf[u] += s / (float)m;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (k < m && l[k] >= 0) s += g[k];
+        vst<CF>(xf + u, vld<CF>(xf + u) + s / (float)m);
     }
 }
 
@@ -220,30 +362,37 @@ __device__ __forceinline__ bool gsync(const GridSync &gs, unsigned &gen, bool ok
 __device__ __forceinline__ bool vsync(const GridSync &gs, unsigned &gen, bool ok, int nbr, bool use_nbr)
 {
     if (!use_nbr) return ok && grid_sync(gs, gen);
-    return ok && nbr_sync(gs, gen, nbr);
+    if constexpr (kCO) return ok && nbr_sync_coherent(gs, gen, nbr);
+    else return ok && nbr_sync(gs, gen, nbr);
 }
 
-// A coarse level as seen by one workgroup: its vectors live in the
-// workgroup's LDS block at lds_off (x, t, b; the residual reuses the sweep
-// buffer that is free after the pre-sweeps); the matrix stays in HBM.
-__device__ __forceinline__ MgLevelDev lds_level(const MgLevelDev &G, float *mgl, int pre)
+// Copy of the LDS levels' matrices into the workgroup's LDS block.
+__device__ __forceinline__ void lds_load_levels(const MixedParams &P, const MgLds &S)
 {
-    MgLevelDev A = G;
-    A.x = mgl + G.lds_off;
-    A.t = A.x + G.n;
-    A.b = A.t + G.n;
-    A.r = sweep_buf(A, pre - 1);
-    return A;
+    for (int l = 0; l + 1 < P.nmg; l++) {
+        const MCL_L MgLevelDev *G = S.lev + l;
+        if (!G->block) continue;
+        const int m = G->WK * G->n, moff = G->moff, doff = G->lds_off / 3, n = G->n;
+        const MCL_G float *val = gp(G->val), *dinv = gp(G->dinv);
+        const MCL_G unsigned short *col = gp(G->col16);
+        for (int e = threadIdx.x; e < m; e += blockDim.x) {
+            S.val[moff + e] = val[e];
+            S.col[moff + e] = col[e];
+        }
+        for (int i = threadIdx.x; i < n; i += blockDim.x) S.dinv[doff + i] = dinv[i];
+    }
+    __syncthreads();
 }
 
 // Coarsest level: x = K0^-1 b by the dense FP64 inverse, four lanes per row
 // (MCL_BLOCK >= 4 n threads; n <= 64).
-__device__ __forceinline__ void coarse_solve(const MixedParams &P, const MgLevelDev &C)
+__device__ __forceinline__ void coarse_solve(const MixedParams &P, const LdsLevel &C)
 {
     const int i = threadIdx.x >> 2, q = threadIdx.x & 3;
+    const MCL_G double *cinv = gp(P.cinv);
     double s = 0.0;
     if (i < C.n)
-        for (int j = q; j < C.n; j += 4) s += P.cinv[i * C.n + j] * (double)C.b[j];
+        for (int j = q; j < C.n; j += 4) s += cinv[i * C.n + j] * (double)C.b[j];
     s += __shfl_xor(s, 1, MCL_WAVE);
     s += __shfl_xor(s, 2, MCL_WAVE);
     if (q == 0 && i < C.n) sweep_buf(C, 2 * P.pre)[i] = (float)s;
@@ -252,30 +401,33 @@ __device__ __forceinline__ void coarse_solve(const MixedParams &P, const MgLevel
 // The V-cycle after the finest level's right-hand side and first sweep are in
 // place and synchronised, up to (not including) the finest level's last
 // post-sweep, which the caller fuses with its dot product.
-//  * grid levels (lb+1 .. top): rows spread over the grid, grid barriers;
+//  * grid levels (lb+1 .. top): rows spread over the grid, neighbour waits;
 //  * coarse levels (0 .. lb, `block` set): every workgroup runs them itself,
-//    identically, with its vectors in LDS and workgroup barriers only, so no
-//    workgroup waits on another and level lb+1 prolongates from its own LDS.
-__device__ __forceinline__ void vcycle_rest(const MixedParams &P, const float *lam, float *mgl, const GridSync &gs,
-                                            unsigned &gen, bool &ok, int nbr)
+//    identically, with its vectors and matrices in LDS and workgroup barriers
+//    only, so no workgroup waits on another and level lb+1 prolongates from
+//    its own LDS.
+__device__ __forceinline__ void vcycle_rest(const MixedParams &P, const MgLds &mgl, const GridSync &gs, unsigned &gen,
+                                            bool &ok, int nbr)
 {
     const int top = P.nmg - 1, pre = P.pre, kf = 2 * P.pre;
     const bool un = P.use_nbr != 0;
+    const MCL_L float *lam = mgl.lam;
     int lb = 0;
     for (int l = 0; l < top; l++)
-        if (P.mg[l].block) lb = l;
+        if (mgl.lev[l].block) lb = l;
     for (int l = top; l > lb; l--) {
-        const MgLevelDev &A = P.mg[l];
+        const GLevel A = glevel(mgl.lev + l);
         const Ex ex = own_rows(A.n);
         const float w = P.omega / lam[l];
         for (int k = 2; k <= pre; k++) {
-            jacobi(A, k, w, ex);
+            jacobi<kCO>(A, k, w, ex);
             ok = vsync(gs, gen, ok, nbr, un);
         }
-        residual(A, sweep_buf(A, pre), ex);
+        residual<kCO>(A, sweep_buf(A, pre), ex);
         if (l - 1 > lb) {
             ok = vsync(gs, gen, ok, nbr, un);
-            restrict_to(P.mg[l - 1], A, true, P.omega / lam[l - 1], own_rows(P.mg[l - 1].n));
+            const GLevel C = glevel(mgl.lev + l - 1);
+            restrict_to<kCO, kCO>(C, A, true, P.omega / lam[l - 1], own_rows(C.n));
             ok = vsync(gs, gen, ok, nbr, un);
         } else {
             // every workgroup restricts the whole residual into its LDS
@@ -284,53 +436,57 @@ __device__ __forceinline__ void vcycle_rest(const MixedParams &P, const float *l
     }
     MCL_PHASE(2);
     {
-        MgLevelDev B = lds_level(P.mg[lb], mgl, pre);
-        restrict_to(B, P.mg[lb + 1], lb > 0, P.omega / lam[lb], wg_rows(B.n));
+        const LdsLevel B = lds_level(mgl, lb, pre);
+        restrict_to<false, false>(B, glevel(mgl.lev + lb + 1), lb > 0, P.omega / lam[lb], wg_rows(B.n));
         __syncthreads();
+        MCL_PHASE(7);
         for (int l = lb; l > 0; l--) {
-            const MgLevelDev A = lds_level(P.mg[l], mgl, pre);
+            const LdsLevel A = lds_level(mgl, l, pre);
             const Ex ex = wg_rows(A.n);
             const float w = P.omega / lam[l];
             for (int k = 2; k <= pre; k++) {
-                jacobi(A, k, w, ex);
+                jacobi<false>(A, k, w, ex);
                 __syncthreads();
             }
-            residual(A, sweep_buf(A, pre), ex);
+            residual<false>(A, sweep_buf(A, pre), ex);
             __syncthreads();
-            const MgLevelDev C = lds_level(P.mg[l - 1], mgl, pre);
-            restrict_to(C, A, l - 1 > 0, P.omega / lam[l - 1], wg_rows(C.n));
+            const LdsLevel C = lds_level(mgl, l - 1, pre);
+            restrict_to<false, false>(C, A, l - 1 > 0, P.omega / lam[l - 1], wg_rows(C.n));
             __syncthreads();
         }
-        coarse_solve(P, lds_level(P.mg[0], mgl, pre));
+        MCL_PHASE(8);
+        coarse_solve(P, lds_level(mgl, 0, pre));
         __syncthreads();
+        MCL_PHASE(9);
         for (int l = 1; l <= lb; l++) {
-            const MgLevelDev A = lds_level(P.mg[l], mgl, pre);
-            const MgLevelDev C = lds_level(P.mg[l - 1], mgl, pre);
+            const LdsLevel A = lds_level(mgl, l, pre);
+            const LdsLevel C = lds_level(mgl, l - 1, pre);
             const Ex ex = wg_rows(A.n);
-            prolong_add(A, sweep_buf(A, pre), C, sweep_buf(C, kf), ex);
+            prolong_add<false, false>(A, sweep_buf(A, pre), sweep_buf(C, kf), ex);
             __syncthreads();
             const float w = P.omega / lam[l];
             for (int k = pre + 1; k <= kf; k++) {
-                jacobi(A, k, w, ex);
+                jacobi<false>(A, k, w, ex);
                 __syncthreads();
             }
         }
     }
     MCL_PHASE(3);
     for (int l = lb + 1; l <= top; l++) {
-        const MgLevelDev &A = P.mg[l];
+        const GLevel A = glevel(mgl.lev + l);
         const Ex ex = own_rows(A.n);
         if (l - 1 == lb) {
-            const MgLevelDev C = lds_level(P.mg[lb], mgl, pre);
-            prolong_add(A, sweep_buf(A, pre), C, sweep_buf(C, kf), ex);
+            const LdsLevel C = lds_level(mgl, lb, pre);
+            prolong_add<kCO, false>(A, sweep_buf(A, pre), sweep_buf(C, kf), ex);
         } else {
-            prolong_add(A, sweep_buf(A, pre), P.mg[l - 1], sweep_buf(P.mg[l - 1], kf), ex);
+            const GLevel C = glevel(mgl.lev + l - 1);
+            prolong_add<kCO, kCO>(A, sweep_buf(A, pre), sweep_buf(C, kf), ex);
         }
         ok = vsync(gs, gen, ok, nbr, un);
         const float w = P.omega / lam[l];
         const int kend = l == top ? kf - 1 : kf;
         for (int k = pre + 1; k <= kend; k++) {
-            jacobi(A, k, w, ex);
+            jacobi<kCO>(A, k, w, ex);
             ok = vsync(gs, gen, ok, nbr, un);
         }
     }
@@ -339,19 +495,19 @@ __device__ __forceinline__ void vcycle_rest(const MixedParams &P, const float *l
 
 // Finest level's last post-sweep z = x_{2 pre}, fused with sum r_i z_i
 // (published to reduction slot `slot`).
-__device__ __forceinline__ void last_sweep_rz(const MixedParams &P, const float *lam, const double *__restrict__ r,
+__device__ __forceinline__ void last_sweep_rz(const MixedParams &P, const MgLds &mgl, const double *__restrict__ r,
                                               int slot, dd (*lds)[MCL_BLOCK / MCL_WAVE])
 {
-    const MgLevelDev &A = P.mg[P.nmg - 1];
-    const float w = P.omega / lam[P.nmg - 1];
+    const GLevel A = glevel(mgl.lev + P.nmg - 1);
+    const float w = P.omega / mgl.lam[P.nmg - 1];
     const int kf = 2 * P.pre;
-    const float *__restrict__ in = sweep_buf(A, kf - 1);
-    float *__restrict__ out = sweep_buf(A, kf);
+    const auto in = sweep_buf(A, kf - 1);
+    const auto out = sweep_buf(A, kf);
     dd acc[1] = {dd_zero()};
     const Ex ex = own_rows(A.n);
     const LevelDev &L = P.U.L;
     for (int i = ex.b; i < ex.e; i += ex.s) {
-        float zi = in[i] + w * A.dinv[i] * (A.b[i] - ell32(A, i, in));
+        float zi = vld<kCO>(in + i) + w * A.dinv[i] * (vld<kCO>(A.b + i) - ell32<kCO>(A, i, in));
         if (L.nh > 0 && L.hp_n[i] != 0) zi = 0.f;  // z lives in the constrained space
         out[i] = zi;
         acc[0] = dd_add_prod(acc[0], r[i], (double)zi);
@@ -408,16 +564,16 @@ __device__ __forceinline__ double ct_gather(const LevelDev &L, int i, double yi,
 //   (3) rest of the V-cycle, last sweep fused with r.z          | barrier
 // Reduction slots: 0,1 (start), 2 (r.z), 12 (d.q), 13,14 (r.r, r.z).
 // Returns the iteration count.
-__device__ __forceinline__ int inner_mixed(const MixedParams &P, const float *lam, const double *__restrict__ b,
+__device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *__restrict__ b,
                                            double *__restrict__ x, bool x_zero, const GridSync &gs,
                                            unsigned &gen, bool &ok, dd (*lds)[MCL_BLOCK / MCL_WAVE],
-                                           float *mgl, long long &spmv, double floor2, int nbr)
+                                           const MgLds &mgl, long long &spmv, double floor2, int nbr)
 {
     const LevelDev &L = P.U.L;
     const bool use_mg = P.nmg > 1;
-    const MgLevelDev &A = P.mg[P.nmg - 1];
-    const float wtop = use_mg ? P.omega / lam[P.nmg - 1] : 0.f;
-    float *__restrict__ zf = use_mg ? sweep_buf(A, 2 * P.pre) : A.x;
+    const GLevel A = glevel(mgl.lev + P.nmg - 1);
+    const float wtop = use_mg ? P.omega / mgl.lam[P.nmg - 1] : 0.f;
+    const MCL_G float *zf = use_mg ? sweep_buf(A, 2 * P.pre) : A.x;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsize = gridDim.x * blockDim.x;
     const int n = L.neq;
     double *__restrict__ r = P.rr, *__restrict__ q = P.U.cg_q;
@@ -459,8 +615,8 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const float *la
     const double stop2 = fmax(P.inner_rtol * P.inner_rtol * bn2, floor2);
     if (!(bn2 > stop2)) return 0;   // x = 0 (also on hanging equations)
     if (use_mg) {
-        vcycle_rest(P, lam, mgl, gs, gen, ok, nbr);
-        last_sweep_rz(P, lam, r, 2, lds);
+        vcycle_rest(P, mgl, gs, gen, ok, nbr);
+        last_sweep_rz(P, mgl, r, 2, lds);
         ok = ok && grid_sync(gs, gen);
         double o[1];
         reduce_collect<1>(P.U.rb, 2, gridDim.x, o, lds);
@@ -562,8 +718,8 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const float *la
         it++;
         if (rr <= stop2) break;
         if (use_mg) {
-            vcycle_rest(P, lam, mgl, gs, gen, ok, nbr);
-            last_sweep_rz(P, lam, r, 2, lds);
+            vcycle_rest(P, mgl, gs, gen, ok, nbr);
+            last_sweep_rz(P, mgl, r, 2, lds);
             ok = ok && grid_sync(gs, gen);
             double o[1];
             reduce_collect<1>(P.U.rb, 2, gridDim.x, o, lds);
@@ -591,10 +747,23 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const float *la
 __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
 {
     __shared__ dd lds[4][MCL_BLOCK / MCL_WAVE];
-    __shared__ float mgl[3 * MCL_MG_LDS_ROWS];
+    __shared__ float mg_vec[3 * MCL_MG_LDS_ROWS], mg_dinv[MCL_MG_LDS_ROWS], mg_val[MCL_MG_LDS_NNZ];
+    __shared__ unsigned short mg_col[MCL_MG_LDS_NNZ];
+    // level descriptors and Gershgorin bounds (read in every phase; LDS keeps
+    // them out of the grid barriers' cache invalidations)
+    __shared__ MgLevelDev mg_lev[MCL_MG_MAX_LEVELS];
+    __shared__ float mg_lam[MCL_MG_MAX_LEVELS];
+    {
+        const int nw = P.nmg * (int)(sizeof(MgLevelDev) / sizeof(int));
+        for (int e = threadIdx.x; e < nw; e += blockDim.x) ((int *)mg_lev)[e] = ((const int *)P.mg)[e];
+        for (int e = threadIdx.x; e < P.nmg; e += blockDim.x) mg_lam[e] = P.lam[e];
+        __syncthreads();
+    }
+    const MgLds mgl{(const MCL_L MgLevelDev *)mg_lev, (const MCL_L float *)mg_lam, (MCL_L float *)mg_vec,
+                    (MCL_L float *)mg_dinv, (MCL_L float *)mg_val, (MCL_L unsigned short *)mg_col};
+    if (P.nmg > 1) lds_load_levels(P, mgl);
     const UzawaParams &U = P.U;
     const LevelDev &L = U.L;
-    const float *lam = P.lam;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsize = gridDim.x * blockDim.x;
     const int neq = L.neq, nel = L.nel;
     unsigned gen = __hip_atomic_load(U.gs.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
@@ -646,7 +815,7 @@ __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
         ff = o[0];
     }
     MCL_PHASE(6);
-    inner += inner_mixed(P, lam, U.Ahv, U.u1, true, U.gs, gen, ok, lds, mgl, spmv,
+    inner += inner_mixed(P, U.Ahv, U.u1, true, U.gs, gen, ok, lds, mgl, spmv,
                          P.inner_rtol * P.inner_rtol * ff, nbr);
     for (int i = gtid; i < neq; i += gsize) U.d[i] += U.u1[i];
     ok = ok && grid_sync(U.gs, gen);
@@ -703,7 +872,7 @@ __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
         }
         ok = ok && grid_sync(U.gs, gen);
         MCL_PHASE(6);
-        inner += inner_mixed(P, lam, U.Ahv, U.u1, true, U.gs, gen, ok, lds, mgl, spmv, 0.0, nbr);
+        inner += inner_mixed(P, U.Ahv, U.u1, true, U.gs, gen, ok, lds, mgl, spmv, 0.0, nbr);
         {
             dd acc[1] = {dd_zero()};
             for (int e = gtid; e < nel; e += gsize) {
