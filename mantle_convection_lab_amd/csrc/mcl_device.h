@@ -144,6 +144,34 @@ __device__ __forceinline__ bool nbr_sync_coherent(const GridSync &gs, unsigned &
     return s_abort == 0;
 }
 
+// Grid barrier for phases whose cross-workgroup data is accessed only with
+// agent-scope atomic loads and stores: the flag protocol of grid_sync with the
+// ordering of nbr_sync_coherent (every wave's memory operations complete
+// before the arrival; loads after the release are coherent), so no L2
+// write-back or invalidate.
+__device__ __forceinline__ bool grid_sync_coherent(const GridSync &gs, unsigned &my_gen)
+{
+    __shared__ unsigned s_abort;
+    const unsigned target = my_gen + 1;
+    my_gen = target;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        __hip_atomic_store(gs.arrive + blockIdx.x, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (blockIdx.x == 0) {
+        for (unsigned b = threadIdx.x; b < gs.nblocks; b += blockDim.x) spin_until(gs.arrive + b, target, gs.abort);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_store(gs.gen, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (threadIdx.x == 0) {
+        spin_until(gs.gen, target, gs.abort);
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (threadIdx.x == 0) s_abort = __hip_atomic_load(gs.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    return s_abort == 0;
+}
+
 // ------------------------------------------------------- compensated reduction
 // A double-double value (hi + lo).  Sums use TwoSum, products TwoProduct via
 // fma, so a dot product is accumulated as if in ~106-bit precision and rounded
@@ -214,7 +242,9 @@ __device__ __forceinline__ void block_reduce_dd(dd (&v)[NR], dd (*lds)[MCL_BLOCK
 
 // Partial-sum exchange through global memory: every workgroup writes its
 // partials, and after a grid barrier every workgroup reduces all of them in the
-// same order, so every workgroup obtains the identical value.
+// same order, so every workgroup obtains the identical value.  The partials
+// are stored and loaded with agent-scope atomics, so either kind of grid
+// barrier (grid_sync or grid_sync_coherent) orders them.
 struct RedBuf {
     dd *part;      // [slot][MCL_MAX_WG]
     double *pmax;  // [slot][MCL_MAX_WG]
@@ -227,7 +257,11 @@ __device__ __forceinline__ void reduce_publish(const RedBuf &rb, int slot0, dd (
     block_reduce_dd<NR>(v, lds);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int r = 0; r < NR; r++) rb.part[(slot0 + r) * MCL_MAX_WG + blockIdx.x] = v[r];
+        for (int r = 0; r < NR; r++) {
+            dd *p = rb.part + (slot0 + r) * MCL_MAX_WG + blockIdx.x;
+            __hip_atomic_store(&p->hi, v[r].hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&p->lo, v[r].lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
 }
 
 template <int NR>
@@ -239,7 +273,12 @@ __device__ __forceinline__ void reduce_collect(const RedBuf &rb, int slot0, int 
     for (int r = 0; r < NR; r++) v[r] = dd_zero();
     for (int b = threadIdx.x; b < nblocks; b += MCL_BLOCK)
 #pragma unroll
-        for (int r = 0; r < NR; r++) v[r] = dd_add(v[r], rb.part[(slot0 + r) * MCL_MAX_WG + b]);
+        for (int r = 0; r < NR; r++) {
+            const dd *p = rb.part + (slot0 + r) * MCL_MAX_WG + b;
+            const dd t{__hip_atomic_load(&p->hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                       __hip_atomic_load(&p->lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
+            v[r] = dd_add(v[r], t);
+        }
     block_reduce_dd<NR>(v, lds);
 #pragma unroll
     for (int r = 0; r < NR; r++) out[r] = v[r].hi + v[r].lo;

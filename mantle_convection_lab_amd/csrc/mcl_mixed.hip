@@ -154,6 +154,7 @@ constexpr bool kCO = MCL_VCYCLE_COHERENT != 0;
 template <bool CO, typename Ptr>
 __device__ __forceinline__ float vld(Ptr p)
 {
+    static_assert(sizeof(*p) == sizeof(float), "vld reads float vectors (vldd: double)");
     if constexpr (CO) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     else return *p;
 }
@@ -161,6 +162,23 @@ __device__ __forceinline__ float vld(Ptr p)
 template <bool CO, typename Ptr>
 __device__ __forceinline__ void vst(Ptr p, float v)
 {
+    static_assert(sizeof(*p) == sizeof(float), "vst writes float vectors (vstd: double)");
+    if constexpr (CO) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+}
+
+template <bool CO, typename Ptr>
+__device__ __forceinline__ double vldd(Ptr p)
+{
+    static_assert(sizeof(*p) == sizeof(double), "vldd reads double vectors");
+    if constexpr (CO) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return *p;
+}
+
+template <bool CO, typename Ptr>
+__device__ __forceinline__ void vstd(Ptr p, double v)
+{
+    static_assert(sizeof(*p) == sizeof(double), "vstd writes double vectors");
     if constexpr (CO) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     else *p = v;
 }
@@ -354,7 +372,14 @@ __device__ __forceinline__ void prolong_add(const LF &F, XF xf, XC xc, Ex ex)
     }
 }
 
-__device__ __forceinline__ bool gsync(const GridSync &gs, unsigned &gen, bool ok) { return ok && grid_sync(gs, gen); }
+// Grid-wide wait between phases whose cross-workgroup data is accessed only
+// with agent-scope atomics (no cache maintenance), or a fenced grid barrier
+// when the V-cycle runs with plain accesses.
+__device__ __forceinline__ bool csync(const GridSync &gs, unsigned &gen, bool ok)
+{
+    if constexpr (kCO) return ok && grid_sync_coherent(gs, gen);
+    else return ok && grid_sync(gs, gen);
+}
 
 // Wait between two stencil phases of the V-cycle: only for the workgroups in
 // this workgroup's neighbour list (thread t polls neighbour t; nbr < 0: none),
@@ -431,13 +456,13 @@ __device__ __forceinline__ void vcycle_rest(const MixedParams &P, const MgLds &m
             ok = vsync(gs, gen, ok, nbr, un);
         } else {
             // every workgroup restricts the whole residual into its LDS
-            ok = gsync(gs, gen, ok);
+            ok = csync(gs, gen, ok);
         }
     }
     MCL_PHASE(2);
     {
         const LdsLevel B = lds_level(mgl, lb, pre);
-        restrict_to<false, false>(B, glevel(mgl.lev + lb + 1), lb > 0, P.omega / lam[lb], wg_rows(B.n));
+        restrict_to<kCO, false>(B, glevel(mgl.lev + lb + 1), lb > 0, P.omega / lam[lb], wg_rows(B.n));
         __syncthreads();
         MCL_PHASE(7);
         for (int l = lb; l > 0; l--) {
@@ -509,7 +534,7 @@ __device__ __forceinline__ void last_sweep_rz(const MixedParams &P, const MgLds 
     for (int i = ex.b; i < ex.e; i += ex.s) {
         float zi = vld<kCO>(in + i) + w * A.dinv[i] * (vld<kCO>(A.b + i) - ell32<kCO>(A, i, in));
         if (L.nh > 0 && L.hp_n[i] != 0) zi = 0.f;  // z lives in the constrained space
-        out[i] = zi;
+        vst<kCO>(out + i, zi);
         acc[0] = dd_add_prod(acc[0], r[i], (double)zi);
     }
     reduce_publish<1>(P.U.rb, slot, acc, lds);
@@ -574,14 +599,18 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
     const GLevel A = glevel(mgl.lev + P.nmg - 1);
     const float wtop = use_mg ? P.omega / mgl.lam[P.nmg - 1] : 0.f;
     const MCL_G float *zf = use_mg ? sweep_buf(A, 2 * P.pre) : A.x;
-    const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsize = gridDim.x * blockDim.x;
     const int n = L.neq;
+    // rows of this thread: the same as on the V-cycle's finest level, so x, r
+    // and q stay private to their thread; d and z are read by neighbours
+    // (agent-scope accesses), and between the phases of an iteration the grid
+    // waits without cache maintenance (grid_sync_coherent)
+    const Ex ex = own_rows(n);
     double *__restrict__ r = P.rr, *__restrict__ q = P.U.cg_q;
-    double *dold = P.U.cg_d, *dnew = P.U.cg_s;
+    MCL_G double *dold = gp(P.U.cg_d), *dnew = gp(P.U.cg_s);
     // r = b - K x, then M r
     {
         dd acc[2] = {dd_zero(), dd_zero()};
-        for (int i = gtid; i < n; i += gsize) {
+        for (int i = ex.b; i < ex.e; i += ex.s) {
             double ri = x_zero ? b[i] : b[i] - ell_row(L.Kcol, L.Kval, L.Kncol[i], n, i, x);
             if (x_zero) x[i] = 0.0;
             if (L.nh > 0) {
@@ -592,11 +621,11 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
             acc[0] = dd_add_prod(acc[0], ri, ri);
             if (use_mg) {
                 float bi = (float)ri;
-                A.b[i] = bi;
-                A.x[i] = wtop * A.dinv[i] * bi;
+                vst<kCO>(A.b + i, bi);
+                vst<kCO>(A.x + i, wtop * A.dinv[i] * bi);
             } else {
                 float zi = (float)((double)A.dinv[i] * ri);
-                A.x[i] = zi;
+                vst<kCO>(A.x + i, zi);
                 acc[1] = dd_add_prod(acc[1], ri, (double)zi);
             }
         }
@@ -617,7 +646,7 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
     if (use_mg) {
         vcycle_rest(P, mgl, gs, gen, ok, nbr);
         last_sweep_rz(P, mgl, r, 2, lds);
-        ok = ok && grid_sync(gs, gen);
+        ok = csync(gs, gen, ok);
         double o[1];
         reduce_collect<1>(P.U.rb, 2, gridDim.x, o, lds);
         delta = o[0];
@@ -633,22 +662,22 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
             if (L.nh > 0) {
                 // q = C^T K C d', d' = z + beta d (zero on hanging equations)
                 auto dv = [&](int j) {
-                    double v = (double)zf[j];
-                    if (!first) v += beta * dold[j];
+                    double v = (double)vld<kCO>(zf + j);
+                    if (!first) v += beta * vldd<kCO>(dold + j);
                     return v;
                 };
-                for (int i = gtid; i < n; i += gsize) {
+                for (int i = ex.b; i < ex.e; i += ex.s) {
                     double qi = 0.0, di = 0.0;
                     if (L.hp_n[i] == 0) {
                         di = dv(i);
                         qi = ct_gather(L, i, k_row_c(L, i, dv), [&](int h) { return k_row_c(L, h, dv); });
                     }
-                    dnew[i] = di;
+                    vstd<kCO>(dnew + i, di);
                     q[i] = qi;
                     acc[0] = dd_add_prod(acc[0], di, qi);
                 }
             } else
-            for (int i = gtid; i < n; i += gsize) {
+            for (int i = ex.b; i < ex.e; i += ex.s) {
                 const int nc = L.Kncol[i];
                 double s = 0.0;
                 int k = 0;
@@ -660,28 +689,35 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
                         c[u] = L.Kcol[(k + u) * n + i];
                         v[u] = L.Kval[(k + u) * n + i];
                     }
+                    float zj[9];
+                    double dj[9];
 #pragma unroll
                     for (int u = 0; u < 9; u++) {
-                        double dj = (double)zf[c[u]];
-                        if (!first) dj += beta * dold[c[u]];
-                        s += v[u] * dj;
+                        zj[u] = vld<kCO>(zf + c[u]);
+                        dj[u] = first ? 0.0 : vldd<kCO>(dold + c[u]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 9; u++) {
+                        double d = (double)zj[u];
+                        if (!first) d += beta * dj[u];
+                        s += v[u] * d;
                     }
                 }
                 for (; k < nc; k++) {
                     const int j = L.Kcol[k * n + i];
-                    double dj = (double)zf[j];
-                    if (!first) dj += beta * dold[j];
+                    double dj = (double)vld<kCO>(zf + j);
+                    if (!first) dj += beta * vldd<kCO>(dold + j);
                     s += L.Kval[k * n + i] * dj;
                 }
-                double di = (double)zf[i];
-                if (!first) di += beta * dold[i];
-                dnew[i] = di;
+                double di = (double)vld<kCO>(zf + i);
+                if (!first) di += beta * vldd<kCO>(dold + i);
+                vstd<kCO>(dnew + i, di);
                 q[i] = s;
                 acc[0] = dd_add_prod(acc[0], di, s);
             }
             spmv++;
             reduce_publish<1>(P.U.rb, 12, acc, lds);
-            ok = ok && grid_sync(gs, gen);
+            ok = csync(gs, gen, ok);
             double o[1];
             reduce_collect<1>(P.U.rb, 12, gridDim.x, o, lds);
             dq = o[0];
@@ -692,23 +728,23 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
         double rr, rz = 0.0;
         {
             dd acc[2] = {dd_zero(), dd_zero()};
-            for (int i = gtid; i < n; i += gsize) {
-                x[i] += alpha * dnew[i];
+            for (int i = ex.b; i < ex.e; i += ex.s) {
+                x[i] += alpha * vldd<kCO>(dnew + i);
                 double ri = r[i] - alpha * q[i];
                 r[i] = ri;
                 acc[0] = dd_add_prod(acc[0], ri, ri);
                 if (use_mg) {
                     float bi = (float)ri;
-                    A.b[i] = bi;
-                    A.x[i] = wtop * A.dinv[i] * bi;
+                    vst<kCO>(A.b + i, bi);
+                    vst<kCO>(A.x + i, wtop * A.dinv[i] * bi);
                 } else {
                     float zi = (float)((double)A.dinv[i] * ri);
-                    A.x[i] = zi;
+                    vst<kCO>(A.x + i, zi);
                     acc[1] = dd_add_prod(acc[1], ri, (double)zi);
                 }
             }
             reduce_publish<2>(P.U.rb, 13, acc, lds);
-            ok = ok && grid_sync(gs, gen);
+            ok = csync(gs, gen, ok);
             double o[2];
             reduce_collect<2>(P.U.rb, 13, gridDim.x, o, lds);
             rr = o[0];
@@ -720,7 +756,7 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
         if (use_mg) {
             vcycle_rest(P, mgl, gs, gen, ok, nbr);
             last_sweep_rz(P, mgl, r, 2, lds);
-            ok = ok && grid_sync(gs, gen);
+            ok = csync(gs, gen, ok);
             double o[1];
             reduce_collect<1>(P.U.rb, 2, gridDim.x, o, lds);
             rz = o[0];
@@ -729,13 +765,15 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
         beta = rz / delta;
         delta = rz;
         first = false;
-        double *t = dold;
+        MCL_G double *t = dold;
         dold = dnew;
         dnew = t;
     }
+    // x was written with plain stores since the last fenced barrier
+    ok = ok && grid_sync(gs, gen);
     if (L.nh > 0) {
         // u = C u_f
-        for (int i = gtid; i < n; i += gsize)
+        for (int i = ex.b; i < ex.e; i += ex.s)
             if (L.hp_n[i] != 0) x[i] = c_expand(L, i, [&](int j) { return x[j]; });
         ok = ok && grid_sync(gs, gen);
     }
