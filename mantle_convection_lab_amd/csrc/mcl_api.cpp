@@ -51,6 +51,7 @@ struct Level {
     int *pn = nullptr, *pcol = nullptr, *rcol = nullptr;
     float *rval = nullptr;
     int rw = 0;
+    std::vector<int> rcol_h;                  // host copy of rcol ([rw][neq], -1 padded)
     // buoyancy row lists
     int *brow_ptr = nullptr, *brow_e = nullptr;
     signed char *brow_xi = nullptr, *brow_eta = nullptr;
@@ -81,6 +82,8 @@ struct mcl_ctx {
     double *resid = nullptr;         // device [3]
     // mixed precision: multigrid level table, smoother bounds, coarse inverse, FP32 CG vectors
     MgLevelDev *mg_dev = nullptr;      // level descriptors of the mixed solve
+    std::vector<MgLevelDev> mg_host;   // host copy (the halo lists are added per workgroup count)
+    std::vector<void *> halo_bufs;     // device allocations of the halo lists
     unsigned *lam_bits = nullptr;
     double *cinv = nullptr;
     double *rr = nullptr;
@@ -737,6 +740,7 @@ static int build_multigrid(mcl_ctx *c)
             const float *fp;
             if (dev_upload(c, C.owned, &cp, rcol)) return MCL_E_DEVICE;
             C.rcol = (int *)cp;
+            C.rcol_h = rcol;
             if (dev_upload(c, C.owned, &fp, rval)) return MCL_E_DEVICE;
             C.rval = (float *)fp;
             C.rw = rw;
@@ -760,12 +764,14 @@ static int build_multigrid(mcl_ctx *c)
     if (maps) {
         rc |= dev_alloc(c, c->owned, &c->mg_dev, nlev);
         if (!rc) rc |= hipMemcpy(c->mg_dev, mg.data(), nlev * sizeof(MgLevelDev), hipMemcpyHostToDevice) != hipSuccess;
+        c->mg_host = mg;
     } else {
         // single level: Jacobi preconditioner from the finest entry's FP32 1/diag
         rc |= dev_alloc(c, c->owned, &c->mg_dev, 1);
         MgLevelDev top = mg[nlev - 1];
         top.block = 0;
         if (!rc) rc |= hipMemcpy(c->mg_dev, &top, sizeof(MgLevelDev), hipMemcpyHostToDevice) != hipSuccess;
+        c->mg_host.assign(1, top);
     }
     rc |= dev_alloc(c, c->owned, &c->lam_bits, MCL_MAX_LEVELS);
     rc |= dev_alloc(c, c->owned, &c->cinv, 64 * 64);
@@ -1160,6 +1166,139 @@ static int mixed_prepare(mcl_ctx *c)
     return 0;
 }
 
+// Halo lists of the fused V-cycle phases for nb workgroups (kernel:
+// fused_up / fused_down), their extra data dependencies reported through
+// add(a, b), and the level descriptors re-uploaded.  A level's phase stays
+// unfused if some workgroup's list would not fit the LDS buffer
+// (MCL_HALO_LDS floats); MCL_MG_FUSE=0 switches the fusion off.
+template <typename T>
+static int halo_upload(mcl_ctx *c, const T **out, const std::vector<T> &h)
+{
+    if (dev_upload(c, c->owned, out, h)) return 1;
+    c->halo_bufs.push_back(c->owned.back());
+    return 0;
+}
+
+template <typename Add>
+static int build_halos(mcl_ctx *c, int nb, Add &add)
+{
+    const int top = c->nlev - 1, lb = c->mg_lb;
+    if ((int)c->mg_host.size() != c->nlev) return 0;
+    // descriptors without lists first (no kernel may still read the old ones)
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (MgLevelDev &M : c->mg_host) {
+        M.fuse_up = M.fuse_down = 0;
+        M.hu_ptr = M.hu_row = M.hd_ptr = M.hd_row = nullptr;
+        M.kl = nullptr;
+        M.rl = nullptr;
+    }
+    HIPCHK(c, hipMemcpy(c->mg_dev, c->mg_host.data(), c->mg_host.size() * sizeof(MgLevelDev), hipMemcpyHostToDevice));
+    for (void *q : c->halo_bufs) {
+        (void)hipFree(q);
+        c->owned.erase(std::remove(c->owned.begin(), c->owned.end(), q), c->owned.end());
+    }
+    c->halo_bufs.clear();
+    const char *fe = getenv("MCL_MG_FUSE");
+    if (fe && atoi(fe) == 0) return 0;
+    std::vector<MgLevelDev> mg = c->mg_host;
+    for (int l = lb + 1; l <= top; l++) {
+        MgLevelDev &M = mg[l];
+        Level &A = c->levels[l];
+        const int n = A.neq, WK = A.WK;
+        if (n == 0) continue;
+        const int R = (n + nb - 1) / nb;
+        const int Rc = (c->levels[l - 1].neq + nb - 1) / nb;
+        // up: halo = K columns outside the workgroup's rows
+        {
+            std::vector<int> ptr(nb + 1, 0), rows, H;
+            std::vector<unsigned short> kl((size_t)WK * n, 0);
+            bool fits = true;
+            for (int g = 0; g < nb && fits; g++) {
+                const int b0 = g * R, e0 = std::min(b0 + R, n);
+                H.clear();
+                for (int s = 0; s < WK; s++)
+                    for (int i = b0; i < e0; i++) {
+                        const int j = A.Kcol_h[(size_t)s * n + i];
+                        if (j < b0 || j >= e0) H.push_back(j);
+                    }
+                std::sort(H.begin(), H.end());
+                H.erase(std::unique(H.begin(), H.end()), H.end());
+                if (R + (long long)H.size() > MCL_HALO_LDS) fits = false;
+                for (int s = 0; s < WK; s++)
+                    for (int i = b0; i < e0; i++) {
+                        const int j = A.Kcol_h[(size_t)s * n + i];
+                        const int loc = (j >= b0 && j < e0)
+                                            ? j - b0
+                                            : R + (int)(std::lower_bound(H.begin(), H.end(), j) - H.begin());
+                        kl[(size_t)s * n + i] = (unsigned short)loc;
+                    }
+                rows.insert(rows.end(), H.begin(), H.end());
+                ptr[g + 1] = (int)rows.size();
+                // x' on own and halo rows reads the coarse parents' values
+                if (l - 1 > lb) {
+                    auto parents = [&](int j) {
+                        for (int a = 0; a < A.interp_n[j]; a++) {
+                            const int cc = A.interp_eq[(size_t)a * n + j];
+                            if (cc >= 0) add(g, cc / Rc);
+                        }
+                    };
+                    for (int i = b0; i < e0; i++) parents(i);
+                    for (int j : H) parents(j);
+                }
+            }
+            if (fits) {
+                if (rows.empty()) rows.push_back(0);
+                if (halo_upload(c, &M.hu_ptr, ptr) || halo_upload(c, &M.hu_row, rows) || halo_upload(c, &M.kl, kl))
+                    return MCL_E_DEVICE;
+                M.fuse_up = 1;
+            }
+        }
+        // down: fine rows that the workgroup's coarse rows restrict from
+        if (l - 1 > lb) {
+            Level &C = c->levels[l - 1];
+            const int nc = C.neq, rw = C.rw;
+            std::vector<int> ptr(nb + 1, 0), rows, F;
+            std::vector<short> rl((size_t)std::max(rw, 1) * std::max(nc, 1), -1);
+            bool fits = nc > 0 && (int)C.rcol_h.size() == rw * nc;
+            for (int g = 0; g < nb && fits; g++) {
+                const int c0 = g * Rc, c1 = std::min(c0 + Rc, nc);
+                F.clear();
+                for (int k = 0; k < rw; k++)
+                    for (int cc = c0; cc < c1; cc++) {
+                        const int u = C.rcol_h[(size_t)k * nc + cc];
+                        if (u >= 0) F.push_back(u);
+                    }
+                std::sort(F.begin(), F.end());
+                F.erase(std::unique(F.begin(), F.end()), F.end());
+                if ((long long)F.size() > MCL_HALO_LDS) fits = false;
+                for (int k = 0; k < rw; k++)
+                    for (int cc = c0; cc < c1; cc++) {
+                        const int u = C.rcol_h[(size_t)k * nc + cc];
+                        if (u >= 0)
+                            rl[(size_t)k * nc + cc] = (short)(std::lower_bound(F.begin(), F.end(), u) - F.begin());
+                    }
+                // the residual on these rows reads their b and the K columns' x
+                for (int u : F) {
+                    add(g, u / R);
+                    for (int s = 0; s < WK; s++) add(g, A.Kcol_h[(size_t)s * n + u] / R);
+                }
+                rows.insert(rows.end(), F.begin(), F.end());
+                ptr[g + 1] = (int)rows.size();
+            }
+            if (fits) {
+                if (rows.empty()) rows.push_back(0);
+                if (halo_upload(c, &M.hd_ptr, ptr) || halo_upload(c, &M.hd_row, rows) ||
+                    halo_upload(c, &mg[l - 1].rl, rl))
+                    return MCL_E_DEVICE;
+                M.fuse_down = 1;
+            }
+        }
+    }
+    HIPCHK(c, hipMemcpy(c->mg_dev, mg.data(), mg.size() * sizeof(MgLevelDev), hipMemcpyHostToDevice));
+    c->mg_host = mg;
+    return 0;
+}
+
 // Neighbour lists for the point-to-point waits of the mixed V-cycle.  With
 // the contiguous ownership owner(l, i) = i / ceil(n_l / nb) of the kernel, a
 // workgroup's neighbours are the workgroups that own a column of one of its
@@ -1196,6 +1335,8 @@ static int build_neighbours(mcl_ctx *c, int nb)
                 }
         }
     }
+    int rc = build_halos(c, nb, add);
+    if (rc) return rc;
     std::vector<int> ptr(nb + 1, 0), idx;
     int maxdeg = 0;
     for (int b = 0; b < nb; b++) {

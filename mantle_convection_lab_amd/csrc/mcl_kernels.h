@@ -134,7 +134,23 @@ struct MgLevelDev {
     const int *rcol;         // [rw][n_coarse]
     const float *rval;       // [rw][n_coarse]
     float *x, *b, *r, *t;    // level vectors
+    // fused V-cycle phases of a grid level, with per-workgroup lists for the
+    // workgroup count of the mixed solve (row ownership of own_rows):
+    //  up: x' = x + P x_c on the workgroup's rows and on its halo (the K
+    //      columns outside its rows), then the first post-sweep from x' in LDS;
+    //  down: the residual on the fine rows its coarse rows restrict from (own
+    //      and halo, into LDS), then the coarse right-hand side.
+    int fuse_up, fuse_down;
+    const int *hu_ptr, *hu_row;        // [nb + 1], halo rows (up)
+    const unsigned short *kl;          // [WK][n] K columns as indices into [own rows | halo]
+    const int *hd_ptr, *hd_row;        // [nb + 1], fine rows restricted from (down)
+    const short *rl;                   // [rw][n] (coarse level) restriction columns as indices into hd_row
 };
+
+// floats of the LDS buffer of the fused V-cycle phases (per workgroup)
+#ifndef MCL_HALO_LDS
+#define MCL_HALO_LDS 2048
+#endif
 
 // Multigrid levels the mixed solve can hold (every workgroup keeps a copy of
 // their descriptors in LDS).

@@ -191,6 +191,10 @@ struct GLevel {
     const MCL_G int *pn, *pcol, *rcol;
     const MCL_G float *rval;
     MCL_G float *x, *b, *r, *t;
+    int fuse_up, fuse_down;
+    const MCL_G int *hu_ptr, *hu_row, *hd_ptr, *hd_row;
+    const MCL_G unsigned short *kl;
+    const MCL_G short *rl;
 };
 
 // A coarse level as seen by one workgroup: vectors x, t, b (3 n floats at
@@ -214,6 +218,7 @@ struct MgLds {
     const MCL_L float *lam;
     MCL_L float *vec, *dinv, *val;
     MCL_L unsigned short *col;
+    MCL_L float *halo;       // [MCL_HALO_LDS] buffer of the fused phases
 };
 
 template <typename LV>
@@ -239,6 +244,14 @@ __device__ __forceinline__ GLevel glevel(const MCL_L MgLevelDev *G)
     A.b = gp(G->b);
     A.r = gp(G->r);
     A.t = gp(G->t);
+    A.fuse_up = G->fuse_up;
+    A.fuse_down = G->fuse_down;
+    A.hu_ptr = gp(G->hu_ptr);
+    A.hu_row = gp(G->hu_row);
+    A.hd_ptr = gp(G->hd_ptr);
+    A.hd_row = gp(G->hd_row);
+    A.kl = gp(G->kl);
+    A.rl = gp(G->rl);
     return A;
 }
 
@@ -264,20 +277,21 @@ __device__ __forceinline__ LdsLevel lds_level(const MgLds &S, int l, int pre)
     return A;
 }
 
-template <bool CO, typename LV, typename XP>
-__device__ __forceinline__ float ell32(const LV &A, int i, XP x)
+// Row i of an FP32 ELL matrix (WK slots, slot-major over n rows, columns col)
+// times x.  Rows of width 18 (the bilinear-quad velocity stencil) issue all
+// their loads at once, other widths nine entries per round trip; the sum keeps
+// the slot order.
+template <bool CO, typename CP, typename VP, typename XP>
+__device__ __forceinline__ float ell32c(int WK, int n, CP col, VP val, int i, XP x)
 {
-    // Rows of width 18 (the bilinear-quad velocity stencil) issue all their
-    // loads at once, other widths nine entries per round trip; the sum keeps
-    // the slot order.
     float s = 0.f;
-    if (A.WK == 18) {
+    if (WK == 18) {
         int c[18];
         float v[18], g[18];
 #pragma unroll
         for (int u = 0; u < 18; u++) {
-            c[u] = A.col[u * A.n + i];
-            v[u] = A.val[u * A.n + i];
+            c[u] = col[u * n + i];
+            v[u] = val[u * n + i];
         }
 #pragma unroll
         for (int u = 0; u < 18; u++) g[u] = vld<CO>(x + c[u]);
@@ -286,21 +300,27 @@ __device__ __forceinline__ float ell32(const LV &A, int i, XP x)
         return s;
     }
     int k = 0;
-    for (; k + 9 <= A.WK; k += 9) {
+    for (; k + 9 <= WK; k += 9) {
         int c[9];
         float v[9], g[9];
 #pragma unroll
         for (int u = 0; u < 9; u++) {
-            c[u] = A.col[(k + u) * A.n + i];
-            v[u] = A.val[(k + u) * A.n + i];
+            c[u] = col[(k + u) * n + i];
+            v[u] = val[(k + u) * n + i];
         }
 #pragma unroll
         for (int u = 0; u < 9; u++) g[u] = vld<CO>(x + c[u]);
 #pragma unroll
         for (int u = 0; u < 9; u++) s += v[u] * g[u];
     }
-    for (; k < A.WK; k++) s += A.val[k * A.n + i] * vld<CO>(x + A.col[k * A.n + i]);
+    for (; k < WK; k++) s += val[k * n + i] * vld<CO>(x + col[k * n + i]);
     return s;
+}
+
+template <bool CO, typename LV, typename XP>
+__device__ __forceinline__ float ell32(const LV &A, int i, XP x)
+{
+    return ell32c<CO>(A.WK, A.n, A.col, A.val, i, x);
 }
 
 // sweep k >= 2: x_k = x_{k-1} + w D^-1 (b - K x_{k-1})
@@ -370,6 +390,81 @@ __device__ __forceinline__ void prolong_add(const LF &F, XF xf, XC xc, Ex ex)
             if (k < m && l[k] >= 0) s += g[k];
         vst<CF>(xf + u, vld<CF>(xf + u) + s / (float)m);
     }
+}
+
+// Fused down-leg phase of grid level F (coarse grid level C): the residual
+// r = b - K x_pre on the fine rows this workgroup's coarse rows restrict from
+// (its own and a halo, into LDS), then b_c = R r and the coarse first sweep
+// x_c = w_c D_c^-1 b_c on its coarse rows.  Same operations in the same order
+// as residual + restrict_to, with one wait fewer.
+template <typename LB>
+__device__ __forceinline__ void fused_down(const GLevel &F, const GLevel &C, LB buf, float wc, int pre)
+{
+    const int h0 = F.hd_ptr[blockIdx.x], nh = F.hd_ptr[blockIdx.x + 1] - h0;
+    const auto x = sweep_buf(F, pre);
+    for (int t = threadIdx.x; t < nh; t += blockDim.x) {
+        const int u = F.hd_row[h0 + t];
+        buf[t] = vld<kCO>(F.b + u) - ell32<kCO>(F, u, x);
+    }
+    __syncthreads();
+    const Ex ex = own_rows(C.n);
+    for (int c = ex.b; c < ex.e; c += ex.s) {
+        float s = 0.f;
+        for (int k0 = 0; k0 < C.rw; k0 += 9) {
+            int l[9];
+            float v[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                const bool in = k0 + k < C.rw;
+                l[k] = in ? C.rl[(k0 + k) * C.n + c] : -1;
+                v[k] = in ? C.rval[(k0 + k) * C.n + c] : 0.f;
+            }
+#pragma unroll
+            for (int k = 0; k < 9; k++)
+                if (l[k] >= 0) s += v[k] * buf[l[k]];
+        }
+        vst<kCO>(C.b + c, s);
+        vst<kCO>(C.x + c, wc * C.dinv[c] * s);
+    }
+}
+
+// Fused up-leg phase of grid level F: x' = x_pre + P x_c on the workgroup's
+// rows and its halo (into LDS; x_c the coarse level's result, CC its access
+// mode), then the first post-sweep x_{pre+1} = x' + w D^-1 (b - K x') on its
+// rows with K's columns mapped into the LDS copy.  Same operations in the
+// same order as prolong_add + jacobi, with one wait fewer.
+template <bool CC, typename XC, typename LB>
+__device__ __forceinline__ void fused_up(const GLevel &F, XC xc, LB buf, float w, int pre)
+{
+    const int R = (F.n + (int)gridDim.x - 1) / (int)gridDim.x, b0 = (int)blockIdx.x * R;
+    const int no = max(0, min(R, F.n - b0));
+    const int h0 = F.hu_ptr[blockIdx.x], nh = F.hu_ptr[blockIdx.x + 1] - h0;
+    const auto xin = sweep_buf(F, pre);
+    for (int t = threadIdx.x; t < R + nh; t += blockDim.x) {
+        if (t >= no && t < R) continue;
+        const int u = t < R ? b0 + t : F.hu_row[h0 + t - R];
+        const int m = F.pn[u];
+        int lc[4];
+        float g[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) lc[k] = F.pcol[k * F.n + u];
+#pragma unroll
+        for (int k = 0; k < 4; k++) g[k] = (k < m && lc[k] >= 0) ? vld<CC>(xc + lc[k]) : 0.f;
+        float v = vld<kCO>(xin + u);
+        if (m != 0) {
+            float s = 0.f;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (k < m && lc[k] >= 0) s += g[k];
+            v = v + s / (float)m;
+        }
+        buf[t] = v;
+    }
+    __syncthreads();
+    const auto out = sweep_buf(F, pre + 1);
+    for (int i = b0 + (int)threadIdx.x; i < b0 + no; i += blockDim.x)
+        vst<kCO>(out + i,
+                 buf[i - b0] + w * F.dinv[i] * (vld<kCO>(F.b + i) - ell32c<false>(F.WK, F.n, F.kl, F.val, i, buf)));
 }
 
 // Grid-wide wait between phases whose cross-workgroup data is accessed only
@@ -448,14 +543,19 @@ __device__ __forceinline__ void vcycle_rest(const MixedParams &P, const MgLds &m
             jacobi<kCO>(A, k, w, ex);
             ok = vsync(gs, gen, ok, nbr, un);
         }
-        residual<kCO>(A, sweep_buf(A, pre), ex);
         if (l - 1 > lb) {
-            ok = vsync(gs, gen, ok, nbr, un);
             const GLevel C = glevel(mgl.lev + l - 1);
-            restrict_to<kCO, kCO>(C, A, true, P.omega / lam[l - 1], own_rows(C.n));
+            if (A.fuse_down) {
+                fused_down(A, C, mgl.halo, P.omega / lam[l - 1], pre);
+            } else {
+                residual<kCO>(A, sweep_buf(A, pre), ex);
+                ok = vsync(gs, gen, ok, nbr, un);
+                restrict_to<kCO, kCO>(C, A, true, P.omega / lam[l - 1], own_rows(C.n));
+            }
             ok = vsync(gs, gen, ok, nbr, un);
         } else {
             // every workgroup restricts the whole residual into its LDS
+            residual<kCO>(A, sweep_buf(A, pre), ex);
             ok = csync(gs, gen, ok);
         }
     }
@@ -500,17 +600,22 @@ __device__ __forceinline__ void vcycle_rest(const MixedParams &P, const MgLds &m
     for (int l = lb + 1; l <= top; l++) {
         const GLevel A = glevel(mgl.lev + l);
         const Ex ex = own_rows(A.n);
-        if (l - 1 == lb) {
-            const LdsLevel C = lds_level(mgl, lb, pre);
-            prolong_add<kCO, false>(A, sweep_buf(A, pre), sweep_buf(C, kf), ex);
-        } else {
-            const GLevel C = glevel(mgl.lev + l - 1);
-            prolong_add<kCO, kCO>(A, sweep_buf(A, pre), sweep_buf(C, kf), ex);
-        }
-        ok = vsync(gs, gen, ok, nbr, un);
         const float w = P.omega / lam[l];
         const int kend = l == top ? kf - 1 : kf;
-        for (int k = pre + 1; k <= kend; k++) {
+        // the fused phase makes sweep pre + 1 (the finest level's last sweep
+        // is the caller's, so there it needs pre >= 2)
+        const bool fuse = A.fuse_up && kend >= pre + 1;
+        if (l - 1 == lb) {
+            const LdsLevel C = lds_level(mgl, lb, pre);
+            if (fuse) fused_up<false>(A, sweep_buf(C, kf), mgl.halo, w, pre);
+            else prolong_add<kCO, false>(A, sweep_buf(A, pre), sweep_buf(C, kf), ex);
+        } else {
+            const GLevel C = glevel(mgl.lev + l - 1);
+            if (fuse) fused_up<kCO>(A, sweep_buf(C, kf), mgl.halo, w, pre);
+            else prolong_add<kCO, kCO>(A, sweep_buf(A, pre), sweep_buf(C, kf), ex);
+        }
+        ok = vsync(gs, gen, ok, nbr, un);
+        for (int k = fuse ? pre + 2 : pre + 1; k <= kend; k++) {
             jacobi<kCO>(A, k, w, ex);
             ok = vsync(gs, gen, ok, nbr, un);
         }
@@ -791,6 +896,7 @@ __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
     // them out of the grid barriers' cache invalidations)
     __shared__ MgLevelDev mg_lev[MCL_MG_MAX_LEVELS];
     __shared__ float mg_lam[MCL_MG_MAX_LEVELS];
+    __shared__ float mg_halo[MCL_HALO_LDS];
     {
         const int nw = P.nmg * (int)(sizeof(MgLevelDev) / sizeof(int));
         for (int e = threadIdx.x; e < nw; e += blockDim.x) ((int *)mg_lev)[e] = ((const int *)P.mg)[e];
@@ -798,7 +904,8 @@ __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
         __syncthreads();
     }
     const MgLds mgl{(const MCL_L MgLevelDev *)mg_lev, (const MCL_L float *)mg_lam, (MCL_L float *)mg_vec,
-                    (MCL_L float *)mg_dinv, (MCL_L float *)mg_val, (MCL_L unsigned short *)mg_col};
+                    (MCL_L float *)mg_dinv, (MCL_L float *)mg_val, (MCL_L unsigned short *)mg_col,
+                    (MCL_L float *)mg_halo};
     if (P.nmg > 1) lds_load_levels(P, mgl);
     const UzawaParams &U = P.U;
     const LevelDev &L = U.L;
