@@ -1167,14 +1167,24 @@ static int mixed_prepare(mcl_ctx *c)
 }
 
 // Halo lists of the fused V-cycle phases for nb workgroups (kernel:
-// fused_up / fused_down), their extra data dependencies reported through
-// add(a, b), and the level descriptors re-uploaded.  A level's phase stays
-// unfused if some workgroup's list would not fit the LDS buffer
-// (MCL_HALO_LDS floats); MCL_MG_FUSE=0 switches the fusion off.
+// fused_up / fused_down, deep_up / deep_down), their extra data dependencies
+// reported through add(a, b), and the level descriptors re-uploaded.  On grid
+// level l a workgroup owns rows O = [b0, e0) (b0 = g R, R = ceil(n / nb)).
+// Local row lists keep an inner set as a prefix of an outer one, so a local
+// index into the inner set is valid in the outer set:
+//   up:   W = [O | H1], H1 = K columns of O outside O;
+//         V = [W | H2], H2 = K columns of W outside W;
+//   down: F = the fine rows that the workgroup's coarse rows restrict from;
+//         T = [O | Th], Th = K columns of F outside O;
+//         U = [T | Uh], Uh = K columns of T outside T.
+// (O is indexed b0 .. b0 + R - 1 even where the last workgroup owns fewer.)
+// A phase stays in its unfused form if some workgroup's lists would not fit
+// the LDS buffer (MCL_HALO_LDS floats); MCL_MG_FUSE=0 switches fusion off,
+// MCL_MG_FUSE=1 keeps the one-sweep fusion only.
 template <typename T>
 static int halo_upload(mcl_ctx *c, const T **out, const std::vector<T> &h)
 {
-    if (dev_upload(c, c->owned, out, h)) return 1;
+    if (dev_upload(c, c->owned, out, h.empty() ? std::vector<T>(1) : h)) return 1;
     c->halo_bufs.push_back(c->owned.back());
     return 0;
 }
@@ -1187,9 +1197,10 @@ static int build_halos(mcl_ctx *c, int nb, Add &add)
     // descriptors without lists first (no kernel may still read the old ones)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (MgLevelDev &M : c->mg_host) {
-        M.fuse_up = M.fuse_down = 0;
-        M.hu_ptr = M.hu_row = M.hd_ptr = M.hd_row = nullptr;
-        M.kl = nullptr;
+        M.fuse_up = M.fuse_down = M.deep_up = M.deep_down = 0;
+        M.hu_ptr = M.hu_row = M.hd_ptr = M.hd_row = M.h2_ptr = M.h2_row = M.th_ptr = M.th_row = nullptr;
+        M.uh_ptr = M.uh_row = nullptr;
+        M.kl = M.hm = M.mt = M.mf = nullptr;
         M.rl = nullptr;
     }
     HIPCHK(c, hipMemcpy(c->mg_dev, c->mg_host.data(), c->mg_host.size() * sizeof(MgLevelDev), hipMemcpyHostToDevice));
@@ -1199,98 +1210,201 @@ static int build_halos(mcl_ctx *c, int nb, Add &add)
     }
     c->halo_bufs.clear();
     const char *fe = getenv("MCL_MG_FUSE");
-    if (fe && atoi(fe) == 0) return 0;
+    const int fuse_level = fe ? atoi(fe) : 2;
+    if (fuse_level <= 0) return 0;
+    const long long cap = MCL_HALO_LDS;
     std::vector<MgLevelDev> mg = c->mg_host;
+    auto sorted_unique = [](std::vector<int> &v) {
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+    };
+    auto pos = [](const std::vector<int> &v, int j) {
+        auto it = std::lower_bound(v.begin(), v.end(), j);
+        return (it != v.end() && *it == j) ? (int)(it - v.begin()) : -1;
+    };
     for (int l = lb + 1; l <= top; l++) {
         MgLevelDev &M = mg[l];
         Level &A = c->levels[l];
-        const int n = A.neq, WK = A.WK;
+        Level &C = c->levels[l - 1];
+        const int n = A.neq, WK = A.WK, nc = C.neq, rw = C.rw;
         if (n == 0) continue;
-        const int R = (n + nb - 1) / nb;
-        const int Rc = (c->levels[l - 1].neq + nb - 1) / nb;
-        // up: halo = K columns outside the workgroup's rows
-        {
-            std::vector<int> ptr(nb + 1, 0), rows, H;
-            std::vector<unsigned short> kl((size_t)WK * n, 0);
-            bool fits = true;
-            for (int g = 0; g < nb && fits; g++) {
-                const int b0 = g * R, e0 = std::min(b0 + R, n);
-                H.clear();
-                for (int s = 0; s < WK; s++)
-                    for (int i = b0; i < e0; i++) {
-                        const int j = A.Kcol_h[(size_t)s * n + i];
-                        if (j < b0 || j >= e0) H.push_back(j);
-                    }
-                std::sort(H.begin(), H.end());
-                H.erase(std::unique(H.begin(), H.end()), H.end());
-                if (R + (long long)H.size() > MCL_HALO_LDS) fits = false;
-                for (int s = 0; s < WK; s++)
-                    for (int i = b0; i < e0; i++) {
-                        const int j = A.Kcol_h[(size_t)s * n + i];
-                        const int loc = (j >= b0 && j < e0)
-                                            ? j - b0
-                                            : R + (int)(std::lower_bound(H.begin(), H.end(), j) - H.begin());
-                        kl[(size_t)s * n + i] = (unsigned short)loc;
-                    }
-                rows.insert(rows.end(), H.begin(), H.end());
-                ptr[g + 1] = (int)rows.size();
-                // x' on own and halo rows reads the coarse parents' values
-                if (l - 1 > lb) {
-                    auto parents = [&](int j) {
-                        for (int a = 0; a < A.interp_n[j]; a++) {
-                            const int cc = A.interp_eq[(size_t)a * n + j];
-                            if (cc >= 0) add(g, cc / Rc);
-                        }
-                    };
-                    for (int i = b0; i < e0; i++) parents(i);
-                    for (int j : H) parents(j);
+        const int R = (n + nb - 1) / nb, Rc = (nc + nb - 1) / nb;
+        const bool coarse_grid = l - 1 > lb;
+        auto kcol = [&](int i, int s) { return A.Kcol_h[(size_t)s * n + i]; };
+        auto parents = [&](int g, int j) {
+            if (!coarse_grid) return;
+            for (int a = 0; a < A.interp_n[j]; a++) {
+                const int cc = A.interp_eq[(size_t)a * n + j];
+                if (cc >= 0) add(g, cc / Rc);
+            }
+        };
+        std::vector<int> h1_ptr(nb + 1, 0), h1_row, h2_ptr(nb + 1, 0), h2_row;
+        std::vector<unsigned short> kl((size_t)WK * n, 0), hm;
+        std::vector<int> f_ptr(nb + 1, 0), f_row, th_ptr(nb + 1, 0), th_row, uh_ptr(nb + 1, 0), uh_row;
+        std::vector<unsigned short> mt, mf;
+        std::vector<short> rl((size_t)std::max(rw, 1) * std::max(nc, 1), -1);
+        bool up1 = true, up2 = fuse_level >= 2, dn1 = coarse_grid && (int)C.rcol_h.size() == rw * nc && nc > 0;
+        bool dn2 = dn1 && fuse_level >= 2;
+        std::vector<int> H1, H2, F, Th, Uh;
+        for (int g = 0; g < nb; g++) {
+            const int b0 = g * R, e0 = std::min(b0 + R, n);
+            auto own = [&](int j) { return j >= b0 && j < e0; };
+            // up: H1, H2
+            H1.clear();
+            for (int s = 0; s < WK; s++)
+                for (int i = b0; i < e0; i++)
+                    if (!own(kcol(i, s))) H1.push_back(kcol(i, s));
+            sorted_unique(H1);
+            H2.clear();
+            for (int s = 0; s < WK; s++)
+                for (int j : H1) {
+                    const int q = kcol(j, s);
+                    if (!own(q) && pos(H1, q) < 0) H2.push_back(q);
+                }
+            sorted_unique(H2);
+            const long long nW = R + (long long)H1.size(), nV = nW + (long long)H2.size();
+            if (nW > cap) up1 = false;
+            if (nV + nW > cap) up2 = false;
+            // local index in V (-1: missing, which the construction excludes;
+            // checked so that no LDS index can leave its list)
+            auto locV = [&](int j) {
+                if (own(j)) return j - b0;
+                const int p1 = pos(H1, j);
+                if (p1 >= 0) return R + p1;
+                const int p2 = pos(H2, j);
+                return p2 >= 0 ? R + (int)H1.size() + p2 : -1;
+            };
+            for (int s = 0; s < WK; s++)
+                for (int i = b0; i < e0; i++) {
+                    const int q = locV(kcol(i, s));
+                    if (q < 0 || q >= nW) up1 = up2 = false;
+                    kl[(size_t)s * n + i] = (unsigned short)std::max(q, 0);
+                }
+            // hm: H1 rows -> V, slot-major within the workgroup's block
+            for (int s = 0; s < WK; s++)
+                for (int j : H1) {
+                    const int q = locV(kcol(j, s));
+                    if (q < 0 || q >= nV) up2 = false;
+                    hm.push_back((unsigned short)std::max(q, 0));
+                }
+            h1_row.insert(h1_row.end(), H1.begin(), H1.end());
+            h1_ptr[g + 1] = (int)h1_row.size();
+            h2_row.insert(h2_row.end(), H2.begin(), H2.end());
+            h2_ptr[g + 1] = (int)h2_row.size();
+            // up dependencies: the coarse parents of every row x' is formed on
+            // (the level's own vectors were written before the grid barrier
+            // that precedes the LDS levels)
+            for (int i = b0; i < e0; i++) parents(g, i);
+            for (int j : H1) parents(g, j);
+            if (up2)
+                for (int j : H2) parents(g, j);
+            if (!coarse_grid) {
+                // the down phase above the LDS levels uses T = W, U = V and
+                // reads x_1 on V, written in the previous phase
+                for (int j : H1) add(g, j / R);
+                for (int j : H2) add(g, j / R);
+                continue;
+            }
+            // down: F, Th, Uh
+            const int c0 = g * Rc, c1 = std::min(c0 + Rc, nc);
+            F.clear();
+            for (int k = 0; k < rw; k++)
+                for (int cc = c0; cc < c1; cc++) {
+                    const int u = C.rcol_h[(size_t)k * nc + cc];
+                    if (u >= 0) F.push_back(u);
+                }
+            sorted_unique(F);
+            if ((long long)F.size() > cap) dn1 = false;
+            for (int k = 0; k < rw; k++)
+                for (int cc = c0; cc < c1; cc++) {
+                    const int u = C.rcol_h[(size_t)k * nc + cc];
+                    if (u >= 0) rl[(size_t)k * nc + cc] = (short)pos(F, u);
+                }
+            Th.clear();
+            for (int s = 0; s < WK; s++)
+                for (int u : F)
+                    if (!own(kcol(u, s))) Th.push_back(kcol(u, s));
+            sorted_unique(Th);
+            Uh.clear();
+            for (int s = 0; s < WK; s++) {
+                for (int i = b0; i < e0; i++) {
+                    const int q = kcol(i, s);
+                    if (!own(q) && pos(Th, q) < 0) Uh.push_back(q);
+                }
+                for (int j : Th) {
+                    const int q = kcol(j, s);
+                    if (!own(q) && pos(Th, q) < 0) Uh.push_back(q);
                 }
             }
-            if (fits) {
-                if (rows.empty()) rows.push_back(0);
-                if (halo_upload(c, &M.hu_ptr, ptr) || halo_upload(c, &M.hu_row, rows) || halo_upload(c, &M.kl, kl))
+            sorted_unique(Uh);
+            const long long nT = R + (long long)Th.size(), nU = nT + (long long)Uh.size();
+            if (nU + nT + (long long)F.size() > cap) dn2 = false;
+            auto locU = [&](int j) {
+                if (own(j)) return j - b0;
+                const int pt = pos(Th, j);
+                if (pt >= 0) return R + pt;
+                const int pu = pos(Uh, j);
+                return pu >= 0 ? R + (int)Th.size() + pu : -1;
+            };
+            // mt: T rows (O padded to R, then Th) -> U, slot-major in the block
+            for (int s = 0; s < WK; s++) {
+                for (int t = 0; t < R; t++) {
+                    const int i = b0 + t;
+                    const int q = i < e0 ? locU(kcol(i, s)) : 0;
+                    if (q < 0 || q >= nU) dn2 = false;
+                    mt.push_back((unsigned short)std::max(q, 0));
+                }
+                for (int j : Th) {
+                    const int q = locU(kcol(j, s));
+                    if (q < 0 || q >= nU) dn2 = false;
+                    mt.push_back((unsigned short)std::max(q, 0));
+                }
+            }
+            // mf: F rows -> T (K columns of F lie in T by construction)
+            for (int s = 0; s < WK; s++)
+                for (int u : F) {
+                    const int q = locU(kcol(u, s));
+                    if (q < 0 || q >= nT) dn2 = false;
+                    mf.push_back((unsigned short)std::max(q, 0));
+                }
+            f_row.insert(f_row.end(), F.begin(), F.end());
+            f_ptr[g + 1] = (int)f_row.size();
+            th_row.insert(th_row.end(), Th.begin(), Th.end());
+            th_ptr[g + 1] = (int)th_row.size();
+            uh_row.insert(uh_row.end(), Uh.begin(), Uh.end());
+            uh_ptr[g + 1] = (int)uh_row.size();
+            // down dependencies: the residual reads b and x on F's K columns
+            // (one-sweep form); the two-sweep form reads x_1 on U
+            for (int u : F) {
+                add(g, u / R);
+                for (int s2 = 0; s2 < WK; s2++) add(g, kcol(u, s2) / R);
+            }
+            if (dn2)
+                for (int j : Uh) add(g, j / R);
+        }
+        if (up1) {
+            if (halo_upload(c, &M.hu_ptr, h1_ptr) || halo_upload(c, &M.hu_row, h1_row) || halo_upload(c, &M.kl, kl))
+                return MCL_E_DEVICE;
+            M.fuse_up = 1;
+            if (up2) {
+                if (halo_upload(c, &M.h2_ptr, h2_ptr) || halo_upload(c, &M.h2_row, h2_row) ||
+                    halo_upload(c, &M.hm, hm))
                     return MCL_E_DEVICE;
-                M.fuse_up = 1;
+                M.deep_up = 1;
+                if (!coarse_grid) M.deep_down = 1;   // T = W, U = V
             }
         }
-        // down: fine rows that the workgroup's coarse rows restrict from
-        if (l - 1 > lb) {
-            Level &C = c->levels[l - 1];
-            const int nc = C.neq, rw = C.rw;
-            std::vector<int> ptr(nb + 1, 0), rows, F;
-            std::vector<short> rl((size_t)std::max(rw, 1) * std::max(nc, 1), -1);
-            bool fits = nc > 0 && (int)C.rcol_h.size() == rw * nc;
-            for (int g = 0; g < nb && fits; g++) {
-                const int c0 = g * Rc, c1 = std::min(c0 + Rc, nc);
-                F.clear();
-                for (int k = 0; k < rw; k++)
-                    for (int cc = c0; cc < c1; cc++) {
-                        const int u = C.rcol_h[(size_t)k * nc + cc];
-                        if (u >= 0) F.push_back(u);
-                    }
-                std::sort(F.begin(), F.end());
-                F.erase(std::unique(F.begin(), F.end()), F.end());
-                if ((long long)F.size() > MCL_HALO_LDS) fits = false;
-                for (int k = 0; k < rw; k++)
-                    for (int cc = c0; cc < c1; cc++) {
-                        const int u = C.rcol_h[(size_t)k * nc + cc];
-                        if (u >= 0)
-                            rl[(size_t)k * nc + cc] = (short)(std::lower_bound(F.begin(), F.end(), u) - F.begin());
-                    }
-                // the residual on these rows reads their b and the K columns' x
-                for (int u : F) {
-                    add(g, u / R);
-                    for (int s = 0; s < WK; s++) add(g, A.Kcol_h[(size_t)s * n + u] / R);
-                }
-                rows.insert(rows.end(), F.begin(), F.end());
-                ptr[g + 1] = (int)rows.size();
-            }
-            if (fits) {
-                if (rows.empty()) rows.push_back(0);
-                if (halo_upload(c, &M.hd_ptr, ptr) || halo_upload(c, &M.hd_row, rows) ||
-                    halo_upload(c, &mg[l - 1].rl, rl))
+        if (dn1) {
+            if (halo_upload(c, &M.hd_ptr, f_ptr) || halo_upload(c, &M.hd_row, f_row) ||
+                halo_upload(c, &mg[l - 1].rl, rl))
+                return MCL_E_DEVICE;
+            M.fuse_down = 1;
+            if (dn2) {
+                if (halo_upload(c, &M.th_ptr, th_ptr) || halo_upload(c, &M.th_row, th_row) ||
+                    halo_upload(c, &M.uh_ptr, uh_ptr) || halo_upload(c, &M.uh_row, uh_row) ||
+                    halo_upload(c, &M.mt, mt) || halo_upload(c, &M.mf, mf))
                     return MCL_E_DEVICE;
-                M.fuse_down = 1;
+                M.deep_down = 1;
             }
         }
     }

@@ -134,22 +134,33 @@ struct MgLevelDev {
     const int *rcol;         // [rw][n_coarse]
     const float *rval;       // [rw][n_coarse]
     float *x, *b, *r, *t;    // level vectors
-    // fused V-cycle phases of a grid level, with per-workgroup lists for the
-    // workgroup count of the mixed solve (row ownership of own_rows):
-    //  up: x' = x + P x_c on the workgroup's rows and on its halo (the K
-    //      columns outside its rows), then the first post-sweep from x' in LDS;
-    //  down: the residual on the fine rows its coarse rows restrict from (own
-    //      and halo, into LDS), then the coarse right-hand side.
-    int fuse_up, fuse_down;
-    const int *hu_ptr, *hu_row;        // [nb + 1], halo rows (up)
-    const unsigned short *kl;          // [WK][n] K columns as indices into [own rows | halo]
-    const int *hd_ptr, *hd_row;        // [nb + 1], fine rows restricted from (down)
-    const short *rl;                   // [rw][n] (coarse level) restriction columns as indices into hd_row
+    // fused V-cycle phases of a grid level, with per-workgroup row lists for
+    // the workgroup count of the mixed solve (a workgroup owns rows O of
+    // own_rows; W = [O | H1] adds the K columns of O, V = [W | H2] those of W;
+    // F = the fine rows its coarse rows restrict from, T = [O | Th] adds the
+    // K columns of F, U = [T | Uh] those of T):
+    //  fuse_up:   x' = x + P x_c on W, then sweep pre + 1 on O from x' in LDS;
+    //  fuse_down: the residual on F into LDS, then the coarse right-hand side;
+    //  deep_up (pre = 2): x' on V, sweep 3 on W, sweep 4 on O;
+    //  deep_down (pre = 2): sweep 2 on T from x_1 on U, the residual on F,
+    //      the coarse right-hand side (above the LDS levels: T = W, U = V and
+    //      the residual on O).
+    int fuse_up, fuse_down, deep_up, deep_down;
+    const int *hu_ptr, *hu_row;        // [nb + 1], H1
+    const unsigned short *kl;          // [WK][n] K columns of O as local indices (into W, V)
+    const int *h2_ptr, *h2_row;        // [nb + 1], H2
+    const unsigned short *hm;          // H1 rows' K columns -> V, block hu_ptr[g] WK, slot-major
+    const int *hd_ptr, *hd_row;        // [nb + 1], F
+    const short *rl;                   // [rw][n] (coarse level) restriction columns as indices into F
+    const int *th_ptr, *th_row;        // [nb + 1], Th
+    const int *uh_ptr, *uh_row;        // [nb + 1], Uh
+    const unsigned short *mt;          // T rows' K columns -> U, block (g R + th_ptr[g]) WK, slot-major
+    const unsigned short *mf;          // F rows' K columns -> T, block hd_ptr[g] WK, slot-major
 };
 
 // floats of the LDS buffer of the fused V-cycle phases (per workgroup)
 #ifndef MCL_HALO_LDS
-#define MCL_HALO_LDS 2048
+#define MCL_HALO_LDS 4096
 #endif
 
 // Multigrid levels the mixed solve can hold (every workgroup keeps a copy of

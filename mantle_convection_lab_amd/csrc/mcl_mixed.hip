@@ -191,9 +191,9 @@ struct GLevel {
     const MCL_G int *pn, *pcol, *rcol;
     const MCL_G float *rval;
     MCL_G float *x, *b, *r, *t;
-    int fuse_up, fuse_down;
-    const MCL_G int *hu_ptr, *hu_row, *hd_ptr, *hd_row;
-    const MCL_G unsigned short *kl;
+    int fuse_up, fuse_down, deep_up, deep_down;
+    const MCL_G int *hu_ptr, *hu_row, *hd_ptr, *hd_row, *h2_ptr, *h2_row, *th_ptr, *th_row, *uh_ptr, *uh_row;
+    const MCL_G unsigned short *kl, *hm, *mt, *mf;
     const MCL_G short *rl;
 };
 
@@ -246,11 +246,22 @@ __device__ __forceinline__ GLevel glevel(const MCL_L MgLevelDev *G)
     A.t = gp(G->t);
     A.fuse_up = G->fuse_up;
     A.fuse_down = G->fuse_down;
+    A.deep_up = G->deep_up;
+    A.deep_down = G->deep_down;
     A.hu_ptr = gp(G->hu_ptr);
     A.hu_row = gp(G->hu_row);
     A.hd_ptr = gp(G->hd_ptr);
     A.hd_row = gp(G->hd_row);
+    A.h2_ptr = gp(G->h2_ptr);
+    A.h2_row = gp(G->h2_row);
+    A.th_ptr = gp(G->th_ptr);
+    A.th_row = gp(G->th_row);
+    A.uh_ptr = gp(G->uh_ptr);
+    A.uh_row = gp(G->uh_row);
     A.kl = gp(G->kl);
+    A.hm = gp(G->hm);
+    A.mt = gp(G->mt);
+    A.mf = gp(G->mf);
     A.rl = gp(G->rl);
     return A;
 }
@@ -392,6 +403,32 @@ __device__ __forceinline__ void prolong_add(const LF &F, XF xf, XC xc, Ex ex)
     }
 }
 
+// Restriction of the residual held in LDS (rF, indexed through C.rl) to the
+// workgroup's coarse rows, with the coarse first sweep (restrict_to's order).
+template <typename LB>
+__device__ __forceinline__ void restrict_from_lds(const GLevel &C, LB rF, float wc)
+{
+    const Ex ex = own_rows(C.n);
+    for (int c = ex.b; c < ex.e; c += ex.s) {
+        float s = 0.f;
+        for (int k0 = 0; k0 < C.rw; k0 += 9) {
+            int l[9];
+            float v[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                const bool in = k0 + k < C.rw;
+                l[k] = in ? C.rl[(k0 + k) * C.n + c] : -1;
+                v[k] = in ? C.rval[(k0 + k) * C.n + c] : 0.f;
+            }
+#pragma unroll
+            for (int k = 0; k < 9; k++)
+                if (l[k] >= 0) s += v[k] * rF[l[k]];
+        }
+        vst<kCO>(C.b + c, s);
+        vst<kCO>(C.x + c, wc * C.dinv[c] * s);
+    }
+}
+
 // Fused down-leg phase of grid level F (coarse grid level C): the residual
 // r = b - K x_pre on the fine rows this workgroup's coarse rows restrict from
 // (its own and a halo, into LDS), then b_c = R r and the coarse first sweep
@@ -407,25 +444,7 @@ __device__ __forceinline__ void fused_down(const GLevel &F, const GLevel &C, LB 
         buf[t] = vld<kCO>(F.b + u) - ell32<kCO>(F, u, x);
     }
     __syncthreads();
-    const Ex ex = own_rows(C.n);
-    for (int c = ex.b; c < ex.e; c += ex.s) {
-        float s = 0.f;
-        for (int k0 = 0; k0 < C.rw; k0 += 9) {
-            int l[9];
-            float v[9];
-#pragma unroll
-            for (int k = 0; k < 9; k++) {
-                const bool in = k0 + k < C.rw;
-                l[k] = in ? C.rl[(k0 + k) * C.n + c] : -1;
-                v[k] = in ? C.rval[(k0 + k) * C.n + c] : 0.f;
-            }
-#pragma unroll
-            for (int k = 0; k < 9; k++)
-                if (l[k] >= 0) s += v[k] * buf[l[k]];
-        }
-        vst<kCO>(C.b + c, s);
-        vst<kCO>(C.x + c, wc * C.dinv[c] * s);
-    }
+    restrict_from_lds(C, buf, wc);
 }
 
 // Fused up-leg phase of grid level F: x' = x_pre + P x_c on the workgroup's
@@ -465,6 +484,211 @@ __device__ __forceinline__ void fused_up(const GLevel &F, XC xc, LB buf, float w
     for (int i = b0 + (int)threadIdx.x; i < b0 + no; i += blockDim.x)
         vst<kCO>(out + i,
                  buf[i - b0] + w * F.dinv[i] * (vld<kCO>(F.b + i) - ell32c<false>(F.WK, F.n, F.kl, F.val, i, buf)));
+}
+
+// Row `row` of F's FP32 K times a vector held in LDS: the columns come from a
+// local map (slot k of local row t at map[k nloc + t]), the values from the
+// matrix; the sum in slot order, as ell32.
+template <typename MP, typename XP>
+__device__ __forceinline__ float ell32_map(const GLevel &F, MP map, int nloc, int t, int row, XP x)
+{
+    float s = 0.f;
+    if (F.WK == 18) {
+        int c[18];
+        float v[18], g[18];
+#pragma unroll
+        for (int u = 0; u < 18; u++) {
+            c[u] = map[u * nloc + t];
+            v[u] = F.val[u * F.n + row];
+        }
+#pragma unroll
+        for (int u = 0; u < 18; u++) g[u] = x[c[u]];
+#pragma unroll
+        for (int u = 0; u < 18; u++) s += v[u] * g[u];
+        return s;
+    }
+    int k = 0;
+    for (; k + 9 <= F.WK; k += 9) {
+        int c[9];
+        float v[9], g[9];
+#pragma unroll
+        for (int u = 0; u < 9; u++) {
+            c[u] = map[(k + u) * nloc + t];
+            v[u] = F.val[(k + u) * F.n + row];
+        }
+#pragma unroll
+        for (int u = 0; u < 9; u++) g[u] = x[c[u]];
+#pragma unroll
+        for (int u = 0; u < 9; u++) s += v[u] * g[u];
+    }
+    for (; k < F.WK; k++) s += F.val[k * F.n + row] * x[map[k * nloc + t]];
+    return s;
+}
+
+// A workgroup's rows O = [b0, b0 + no) of a grid level (R = own_rows' block).
+struct Own {
+    int R, b0, no;
+};
+
+__device__ __forceinline__ Own own_block(int n)
+{
+    const int R = (n + (int)gridDim.x - 1) / (int)gridDim.x, b0 = (int)blockIdx.x * R;
+    return Own{R, b0, max(0, min(R, n - b0))};
+}
+
+// Two-sweep down-leg phase of grid level F (pre = 2) above a coarse grid
+// level C: x_1 on U into LDS, sweep 2 on T (stored for the workgroup's own
+// rows), the residual on F, then b_c = R r and the coarse first sweep.  The
+// operations and their order are those of jacobi, residual and restrict_to.
+template <typename LB>
+__device__ __forceinline__ void deep_down(const GLevel &F, const GLevel &C, LB buf, float wf, float wc)
+{
+    const Own o = own_block(F.n);
+    const int g = blockIdx.x;
+    const int t0 = F.th_ptr[g], nth = F.th_ptr[g + 1] - t0;
+    const int u0 = F.uh_ptr[g], nuh = F.uh_ptr[g + 1] - u0;
+    const int f0 = F.hd_ptr[g], nf = F.hd_ptr[g + 1] - f0;
+    const int nT = o.R + nth, nU = nT + nuh;
+    const auto xU = buf;
+    const auto xT = buf + nU;
+    const auto rF = xT + nT;
+    auto urow = [&](int t) { return t < o.R ? o.b0 + t : t < nT ? F.th_row[t0 + t - o.R] : F.uh_row[u0 + t - nT]; };
+    const auto x1 = sweep_buf(F, 1);
+    for (int t = threadIdx.x; t < nU; t += blockDim.x)
+        if (t >= o.R || t < o.no) xU[t] = vld<kCO>(x1 + urow(t));
+    __syncthreads();
+    const auto x2 = sweep_buf(F, 2);
+    const MCL_G unsigned short *mt = F.mt + (size_t)((long long)g * o.R + t0) * F.WK;
+    for (int t = threadIdx.x; t < nT; t += blockDim.x) {
+        if (t < o.R && t >= o.no) continue;
+        const int row = urow(t);
+        const float v = xU[t] + wf * F.dinv[row] * (vld<kCO>(F.b + row) - ell32_map(F, mt, nT, t, row, xU));
+        xT[t] = v;
+        if (t < o.R) vst<kCO>(x2 + row, v);
+    }
+    __syncthreads();
+    const MCL_G unsigned short *mf = F.mf + (size_t)f0 * F.WK;
+    for (int t = threadIdx.x; t < nf; t += blockDim.x) {
+        const int row = F.hd_row[f0 + t];
+        rF[t] = vld<kCO>(F.b + row) - ell32_map(F, mf, nf, t, row, xT);
+    }
+    __syncthreads();
+    restrict_from_lds(C, rF, wc);
+}
+
+// x on V = [O | H1 | H2] rows; a local row t of W = [O | H1] times K from V
+// (the map: kl for O rows, hm for H1 rows)
+struct Halo2 {
+    Own o;
+    int h10, nh1, h20, nh2, nW, nV;
+};
+
+__device__ __forceinline__ Halo2 halo2(const GLevel &F)
+{
+    Halo2 h;
+    h.o = own_block(F.n);
+    const int g = blockIdx.x;
+    h.h10 = F.hu_ptr[g];
+    h.nh1 = F.hu_ptr[g + 1] - h.h10;
+    h.h20 = F.h2_ptr[g];
+    h.nh2 = F.h2_ptr[g + 1] - h.h20;
+    h.nW = h.o.R + h.nh1;
+    h.nV = h.nW + h.nh2;
+    return h;
+}
+
+__device__ __forceinline__ int vrow(const GLevel &F, const Halo2 &h, int t)
+{
+    return t < h.o.R ? h.o.b0 + t : t < h.nW ? F.hu_row[h.h10 + t - h.o.R] : F.h2_row[h.h20 + t - h.nW];
+}
+
+template <typename XP>
+__device__ __forceinline__ float kw_row(const GLevel &F, const Halo2 &h, int t, int row, XP xV)
+{
+    if (t < h.o.R) return ell32c<false>(F.WK, F.n, F.kl, F.val, row, xV);
+    return ell32_map(F, F.hm + (size_t)h.h10 * F.WK, h.nh1, t - h.o.R, row, xV);
+}
+
+// Two-sweep down-leg phase of the grid level just above the LDS levels
+// (pre = 2): x_1 on V, sweep 2 on W (stored for the workgroup's own rows),
+// the residual on its own rows (F.r, which every workgroup then restricts
+// into its LDS).
+template <typename LB>
+__device__ __forceinline__ void deep_down_last(const GLevel &F, LB buf, float wf)
+{
+    const Halo2 h = halo2(F);
+    const auto xV = buf;
+    const auto xW = buf + h.nV;
+    const auto x1 = sweep_buf(F, 1);
+    for (int t = threadIdx.x; t < h.nV; t += blockDim.x)
+        if (t >= h.o.R || t < h.o.no) xV[t] = vld<kCO>(x1 + vrow(F, h, t));
+    __syncthreads();
+    const auto x2 = sweep_buf(F, 2);
+    for (int t = threadIdx.x; t < h.nW; t += blockDim.x) {
+        if (t < h.o.R && t >= h.o.no) continue;
+        const int row = vrow(F, h, t);
+        const float v = xV[t] + wf * F.dinv[row] * (vld<kCO>(F.b + row) - kw_row(F, h, t, row, xV));
+        xW[t] = v;
+        if (t < h.o.R) vst<kCO>(x2 + row, v);
+    }
+    __syncthreads();
+    for (int i = h.o.b0 + (int)threadIdx.x; i < h.o.b0 + h.o.no; i += blockDim.x)
+        vst<kCO>(F.r + i, vld<kCO>(F.b + i) - ell32c<false>(F.WK, F.n, F.kl, F.val, i, xW));
+}
+
+// Two-sweep up-leg phase of grid level F (pre = 2): x' = x_2 + P x_c on V
+// (x_c the coarse result, CC its access mode), sweep 3 on W, sweep 4 on the
+// workgroup's own rows, written to F.x (the buffer of x_1/x_3: its
+// neighbours still read x_2 in this phase).  On the finest level sweep 4 is
+// z, zeroed on hanging equations, with sum r_i z_i published to `slot`.  The
+// operations and their order are those of prolong_add, jacobi and
+// last_sweep_rz.
+template <bool CC, typename XC, typename LB>
+__device__ __forceinline__ void deep_up(const GLevel &F, XC xc, LB buf, float w, bool finest, const MixedParams &P,
+                                        const double *__restrict__ r, int slot, dd (*lds)[MCL_BLOCK / MCL_WAVE])
+{
+    const Halo2 h = halo2(F);
+    const auto xV = buf;
+    const auto xW = buf + h.nV;
+    const auto x2 = sweep_buf(F, 2);
+    for (int t = threadIdx.x; t < h.nV; t += blockDim.x) {
+        if (t < h.o.R && t >= h.o.no) continue;
+        const int u = vrow(F, h, t);
+        const int m = F.pn[u];
+        int lc[4];
+        float gv[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) lc[k] = F.pcol[k * F.n + u];
+#pragma unroll
+        for (int k = 0; k < 4; k++) gv[k] = (k < m && lc[k] >= 0) ? vld<CC>(xc + lc[k]) : 0.f;
+        float v = vld<kCO>(x2 + u);
+        if (m != 0) {
+            float s = 0.f;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (k < m && lc[k] >= 0) s += gv[k];
+            v = v + s / (float)m;
+        }
+        xV[t] = v;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < h.nW; t += blockDim.x) {
+        if (t < h.o.R && t >= h.o.no) continue;
+        const int row = vrow(F, h, t);
+        xW[t] = xV[t] + w * F.dinv[row] * (vld<kCO>(F.b + row) - kw_row(F, h, t, row, xV));
+    }
+    __syncthreads();
+    const LevelDev &L = P.U.L;
+    dd acc[1] = {dd_zero()};
+    for (int i = h.o.b0 + (int)threadIdx.x; i < h.o.b0 + h.o.no; i += blockDim.x) {
+        float z = xW[i - h.o.b0] + w * F.dinv[i] * (vld<kCO>(F.b + i) - ell32c<false>(F.WK, F.n, F.kl, F.val, i, xW));
+        if (finest) {
+            if (L.nh > 0 && L.hp_n[i] != 0) z = 0.f;  // z lives in the constrained space
+            acc[0] = dd_add_prod(acc[0], r[i], (double)z);
+        }
+        vst<kCO>(F.x + i, z);
+    }
+    if (finest) reduce_publish<1>(P.U.rb, slot, acc, lds);
 }
 
 // Grid-wide wait between phases whose cross-workgroup data is accessed only
@@ -518,16 +742,57 @@ __device__ __forceinline__ void coarse_solve(const MixedParams &P, const LdsLeve
     if (q == 0 && i < C.n) sweep_buf(C, 2 * P.pre)[i] = (float)s;
 }
 
+// Finest level's last post-sweep z = x_{2 pre}, fused with sum r_i z_i
+// (published to reduction slot `slot`).
+__device__ __forceinline__ void last_sweep_rz(const MixedParams &P, const MgLds &mgl, const double *__restrict__ r,
+                                              int slot, dd (*lds)[MCL_BLOCK / MCL_WAVE])
+{
+    const GLevel A = glevel(mgl.lev + P.nmg - 1);
+    const float w = P.omega / mgl.lam[P.nmg - 1];
+    const int kf = 2 * P.pre;
+    const auto in = sweep_buf(A, kf - 1);
+    const auto out = sweep_buf(A, kf);
+    dd acc[1] = {dd_zero()};
+    const Ex ex = own_rows(A.n);
+    const LevelDev &L = P.U.L;
+    for (int i = ex.b; i < ex.e; i += ex.s) {
+        float zi = vld<kCO>(in + i) + w * A.dinv[i] * (vld<kCO>(A.b + i) - ell32<kCO>(A, i, in));
+        if (L.nh > 0 && L.hp_n[i] != 0) zi = 0.f;  // z lives in the constrained space
+        vst<kCO>(out + i, zi);
+        acc[0] = dd_add_prod(acc[0], r[i], (double)zi);
+    }
+    reduce_publish<1>(P.U.rb, slot, acc, lds);
+}
+
+// Whether a grid level runs its two-sweep phases (needs pre = 2 and the lists).
+__device__ __forceinline__ bool deep_up_on(const MixedParams &P, const GLevel &A) { return P.pre == 2 && A.deep_up; }
+__device__ __forceinline__ bool deep_down_on(const MixedParams &P, const GLevel &A)
+{
+    return P.pre == 2 && A.deep_down;
+}
+
+// Where a grid level's V-cycle result (sweep 2 pre) lies.
+__device__ __forceinline__ MCL_G float *level_result(const MixedParams &P, const GLevel &A)
+{
+    return deep_up_on(P, A) ? A.x : sweep_buf(A, 2 * P.pre);
+}
+
 // The V-cycle after the finest level's right-hand side and first sweep are in
-// place and synchronised, up to (not including) the finest level's last
-// post-sweep, which the caller fuses with its dot product.
+// place and synchronised, through the finest level's last post-sweep z with
+// sum r_i z_i published to reduction slot `slot` (the caller waits on the
+// grid before reading it or z).
 //  * grid levels (lb+1 .. top): rows spread over the grid, neighbour waits;
+//    with pre = 2 each level's down leg (sweep 2, residual, restriction) and
+//    up leg (prolongation, sweeps 3 and 4) run as one phase each over the
+//    workgroup's halos (deep_down / deep_up), else as the one-sweep fused
+//    phases or the plain sequence;
 //  * coarse levels (0 .. lb, `block` set): every workgroup runs them itself,
 //    identically, with its vectors and matrices in LDS and workgroup barriers
 //    only, so no workgroup waits on another and level lb+1 prolongates from
 //    its own LDS.
 __device__ __forceinline__ void vcycle_rest(const MixedParams &P, const MgLds &mgl, const GridSync &gs, unsigned &gen,
-                                            bool &ok, int nbr)
+                                            bool &ok, int nbr, const double *__restrict__ r, int slot,
+                                            dd (*lds)[MCL_BLOCK / MCL_WAVE])
 {
     const int top = P.nmg - 1, pre = P.pre, kf = 2 * P.pre;
     const bool un = P.use_nbr != 0;
@@ -539,6 +804,17 @@ __device__ __forceinline__ void vcycle_rest(const MixedParams &P, const MgLds &m
         const GLevel A = glevel(mgl.lev + l);
         const Ex ex = own_rows(A.n);
         const float w = P.omega / lam[l];
+        if (deep_down_on(P, A)) {
+            if (l - 1 > lb) {
+                deep_down(A, glevel(mgl.lev + l - 1), mgl.halo, w, P.omega / lam[l - 1]);
+                ok = vsync(gs, gen, ok, nbr, un);
+            } else {
+                // every workgroup restricts the whole residual into its LDS
+                deep_down_last(A, mgl.halo, w);
+                ok = csync(gs, gen, ok);
+            }
+            continue;
+        }
         for (int k = 2; k <= pre; k++) {
             jacobi<kCO>(A, k, w, ex);
             ok = vsync(gs, gen, ok, nbr, un);
@@ -601,48 +877,38 @@ __device__ __forceinline__ void vcycle_rest(const MixedParams &P, const MgLds &m
         const GLevel A = glevel(mgl.lev + l);
         const Ex ex = own_rows(A.n);
         const float w = P.omega / lam[l];
-        const int kend = l == top ? kf - 1 : kf;
+        const bool finest = l == top;
+        if (deep_up_on(P, A)) {
+            if (l - 1 == lb) {
+                const LdsLevel C = lds_level(mgl, lb, pre);
+                deep_up<false>(A, sweep_buf(C, kf), mgl.halo, w, finest, P, r, slot, lds);
+            } else {
+                deep_up<kCO>(A, level_result(P, glevel(mgl.lev + l - 1)), mgl.halo, w, finest, P, r, slot, lds);
+            }
+            if (!finest) ok = vsync(gs, gen, ok, nbr, un);
+            continue;
+        }
+        const int kend = finest ? kf - 1 : kf;
         // the fused phase makes sweep pre + 1 (the finest level's last sweep
-        // is the caller's, so there it needs pre >= 2)
+        // is last_sweep_rz's, so there it needs pre >= 2)
         const bool fuse = A.fuse_up && kend >= pre + 1;
         if (l - 1 == lb) {
             const LdsLevel C = lds_level(mgl, lb, pre);
             if (fuse) fused_up<false>(A, sweep_buf(C, kf), mgl.halo, w, pre);
             else prolong_add<kCO, false>(A, sweep_buf(A, pre), sweep_buf(C, kf), ex);
         } else {
-            const GLevel C = glevel(mgl.lev + l - 1);
-            if (fuse) fused_up<kCO>(A, sweep_buf(C, kf), mgl.halo, w, pre);
-            else prolong_add<kCO, kCO>(A, sweep_buf(A, pre), sweep_buf(C, kf), ex);
+            const auto xc = level_result(P, glevel(mgl.lev + l - 1));
+            if (fuse) fused_up<kCO>(A, xc, mgl.halo, w, pre);
+            else prolong_add<kCO, kCO>(A, sweep_buf(A, pre), xc, ex);
         }
         ok = vsync(gs, gen, ok, nbr, un);
         for (int k = fuse ? pre + 2 : pre + 1; k <= kend; k++) {
             jacobi<kCO>(A, k, w, ex);
             ok = vsync(gs, gen, ok, nbr, un);
         }
+        if (finest) last_sweep_rz(P, mgl, r, slot, lds);
     }
     MCL_PHASE(4);
-}
-
-// Finest level's last post-sweep z = x_{2 pre}, fused with sum r_i z_i
-// (published to reduction slot `slot`).
-__device__ __forceinline__ void last_sweep_rz(const MixedParams &P, const MgLds &mgl, const double *__restrict__ r,
-                                              int slot, dd (*lds)[MCL_BLOCK / MCL_WAVE])
-{
-    const GLevel A = glevel(mgl.lev + P.nmg - 1);
-    const float w = P.omega / mgl.lam[P.nmg - 1];
-    const int kf = 2 * P.pre;
-    const auto in = sweep_buf(A, kf - 1);
-    const auto out = sweep_buf(A, kf);
-    dd acc[1] = {dd_zero()};
-    const Ex ex = own_rows(A.n);
-    const LevelDev &L = P.U.L;
-    for (int i = ex.b; i < ex.e; i += ex.s) {
-        float zi = vld<kCO>(in + i) + w * A.dinv[i] * (vld<kCO>(A.b + i) - ell32<kCO>(A, i, in));
-        if (L.nh > 0 && L.hp_n[i] != 0) zi = 0.f;  // z lives in the constrained space
-        vst<kCO>(out + i, zi);
-        acc[0] = dd_add_prod(acc[0], r[i], (double)zi);
-    }
-    reduce_publish<1>(P.U.rb, slot, acc, lds);
 }
 
 // ------------------------------------------- hanging nodes (Galerkin form)
@@ -703,7 +969,7 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
     const bool use_mg = P.nmg > 1;
     const GLevel A = glevel(mgl.lev + P.nmg - 1);
     const float wtop = use_mg ? P.omega / mgl.lam[P.nmg - 1] : 0.f;
-    const MCL_G float *zf = use_mg ? sweep_buf(A, 2 * P.pre) : A.x;
+    const MCL_G float *zf = use_mg ? level_result(P, A) : A.x;
     const int n = L.neq;
     // rows of this thread: the same as on the V-cycle's finest level, so x, r
     // and q stay private to their thread; d and z are read by neighbours
@@ -749,8 +1015,7 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
     const double stop2 = fmax(P.inner_rtol * P.inner_rtol * bn2, floor2);
     if (!(bn2 > stop2)) return 0;   // x = 0 (also on hanging equations)
     if (use_mg) {
-        vcycle_rest(P, mgl, gs, gen, ok, nbr);
-        last_sweep_rz(P, mgl, r, 2, lds);
+        vcycle_rest(P, mgl, gs, gen, ok, nbr, r, 2, lds);
         ok = csync(gs, gen, ok);
         double o[1];
         reduce_collect<1>(P.U.rb, 2, gridDim.x, o, lds);
@@ -859,8 +1124,7 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
         it++;
         if (rr <= stop2) break;
         if (use_mg) {
-            vcycle_rest(P, mgl, gs, gen, ok, nbr);
-            last_sweep_rz(P, mgl, r, 2, lds);
+            vcycle_rest(P, mgl, gs, gen, ok, nbr, r, 2, lds);
             ok = csync(gs, gen, ok);
             double o[1];
             reduce_collect<1>(P.U.rb, 2, gridDim.x, o, lds);
