@@ -152,7 +152,7 @@ typedef struct mcl_control {
     int schur;                 /* enum mcl_schur */
     int precision;             /* enum mcl_precision */
     int warm_start;            /* 1: keep d and p between steps (overrides zero_P) */
-    double inner_rtol;         /* MCL_PREC_MIXED: relative tolerance of each inner solve */
+    double inner_rtol;         /* MCL_PREC_MIXED: relative tolerance of each inner solve (0: 3e-7) */
     int exact_dots;            /* FP64 path: 1 = dot products summed in the reference's sequential
                                   order (bit-for-bit with the reference; slower), 0 = compensated
                                   parallel sums */

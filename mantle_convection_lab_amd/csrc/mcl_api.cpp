@@ -1496,7 +1496,7 @@ static MixedParams mixed_params(mcl_ctx *c, const UzawaParams &P)
     M.cinv = c->cinv;
     M.nc = c->levels[0].neq;
     M.rr = c->rr;
-    M.inner_rtol = k.inner_rtol > 0 ? k.inner_rtol : 1e-7;
+    M.inner_rtol = k.inner_rtol > 0 ? k.inner_rtol : 3e-7;
     M.max_inner = 1000;
     M.probe = 0;
     M.use_nbr = 0;

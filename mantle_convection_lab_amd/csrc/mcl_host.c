@@ -75,7 +75,7 @@ void mclh_default_input(mclh_input *in)
     in->old_T_level = 1;
     in->zero_P = 1;
     /* keys of this build */
-    in->inner_rtol = 1e-7;
+    in->inner_rtol = 3e-7;
     in->T_seed = 20261202ULL;
     in->write_output = 1;
 }
