@@ -150,7 +150,8 @@ def run_path(path, steps, warmup, log):
     for i in range(steps):
         r = model.step()
         recs.append(r)
-        g = f" guess={'net' if r.get('guess_accepted') else 'prev'}" if "guess_accepted" in r else ""
+        g = f" guess={'net' if r.get('guess_accepted') else 'prev'}" if "guess_accepted" in r else \
+            (" guess=skip" if net is not None else "")
         log(f"[bench] {path} step {i}: uzawa={r['uzawa']} conv={r['converged']} inner={r['inner']} "
             f"solve={r['solve_ms']:.2f} ms res={r['res_mom']:.1e}/{r['res_cont']:.1e}{g}")
     model.ctx.synchronize()

@@ -630,6 +630,10 @@ static int block_rows_threshold()
     return e ? atoi(e) : 1024;
 }
 
+static unsigned coop_blocks(mcl_ctx *c, long long n, int rows_per_thread);
+static int build_neighbours(mcl_ctx *c, int nb);
+#define MCL_MIXED_ROWS_PER_THREAD 1
+
 static int build_multigrid(mcl_ctx *c)
 {
     c->have_mg = false;
@@ -779,6 +783,14 @@ static int build_multigrid(mcl_ctx *c)
     if (rc) return fail(c, MCL_E_DEVICE, "multigrid allocation failed");
     c->have_mg = maps;
     c->mixed_ready = false;
+    // the mixed solve's neighbour and halo lists for its default workgroup
+    // count, built here with the rest of the mesh set-up (a different count,
+    // MCL_COOP_BLOCKS, rebuilds them at the first solve)
+    if (maps) {
+        Level &T = c->levels.back();
+        rc = build_neighbours(c, (int)coop_blocks(c, std::max(T.neq, T.nel), MCL_MIXED_ROWS_PER_THREAD));
+        if (rc) return rc;
+    }
     return 0;
 }
 
@@ -1080,7 +1092,6 @@ extern "C" int mcl_velocity_pressure_KGF_matrix(mcl_ctx *c)
 // the workgroup count); the mixed solve takes one (its V-cycle waits are
 // mostly point-to-point, and more workgroups keep more loads in flight).
 #define MCL_FP64_ROWS_PER_THREAD 2
-#define MCL_MIXED_ROWS_PER_THREAD 1
 static unsigned coop_blocks(mcl_ctx *c, long long n, int rows_per_thread)
 {
     const long long per = (long long)rows_per_thread * MCL_BLOCK;

@@ -281,6 +281,10 @@ class Model:
         Uzawa iteration, which always runs to the same tolerance."""
         self.inp = inp
         self.net = net
+        # learned-guess gate: consecutive steps on which the previous solution
+        # beat the network, and steps since the network was last consulted
+        self._net_losses = 0
+        self._net_idle = 0
         self.device = device
         self.ctrl_overrides = ctrl_overrides
         self.mesh = mesh if mesh is not None else Mesh(inp)
@@ -327,8 +331,9 @@ class Model:
         c.viscosity()
         c.velocity_pressure_KGF_matrix()
         guess = None
-        if self.net is not None:
+        if self.net is not None and self._consult_net():
             guess = self.net.guess(c)
+            self._net_losses = 0 if guess[0] else self._net_losses + 1
         st = c.solve_velocity_pressure()
         if self.inp.write_output and self.inp.save_spacing > 0 and self.timestep % self.inp.save_spacing == 0:
             self.write_output()
@@ -342,6 +347,21 @@ class Model:
         self.history.append(rec)
         self.timestep += 1
         return rec
+
+    # after this many consecutive losses the network is consulted only every
+    # NET_RECHECK steps (a consultation costs a network pass and two probe
+    # solves; once the flow settles the previous step's solution wins)
+    NET_LOSSES = 2
+    NET_RECHECK = 8
+
+    def _consult_net(self):
+        if self._net_losses < self.NET_LOSSES:
+            return True
+        self._net_idle += 1
+        if self._net_idle >= self.NET_RECHECK:
+            self._net_idle = 0
+            return True
+        return False
 
     def field(self, name):
         return self.ctx.get_field(self.finest, name)
