@@ -116,9 +116,10 @@ struct Ex {
 };
 
 // Contiguous row ownership of the V-cycle's grid levels: workgroup g owns rows
-// [g R, (g+1) R) with R = ceil(n / gridDim.x), so a workgroup's rows form a
-// compact patch of the Morton-ordered mesh and it exchanges data with few
-// others (the host's neighbour lists use the same formula).
+// [g R, (g+1) R) with R = ceil(n / gridDim.x).  The equations are numbered
+// node line by node line, so a workgroup's rows are about one line of the
+// mesh and it exchanges data with the few workgroups owning the lines next to
+// it (the host's neighbour lists use the same formula).
 __device__ __forceinline__ Ex own_rows(int n)
 {
     const int R = (n + (int)gridDim.x - 1) / (int)gridDim.x;
