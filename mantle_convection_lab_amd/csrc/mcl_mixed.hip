@@ -1052,6 +1052,31 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
                 const int nc = L.Kncol[i];
                 double s = 0.0;
                 int k = 0;
+                if (L.WK == 18) {
+                    // the whole width-18 row in one round trip (padding
+                    // slots hold column i and value 0, adding exact zeros)
+                    int c[18];
+                    double v[18];
+#pragma unroll
+                    for (int u = 0; u < 18; u++) {
+                        c[u] = L.Kcol[u * n + i];
+                        v[u] = L.Kval[u * n + i];
+                    }
+                    float zj[18];
+                    double dj[18];
+#pragma unroll
+                    for (int u = 0; u < 18; u++) {
+                        zj[u] = vld<kCO>(zf + c[u]);
+                        dj[u] = first ? 0.0 : vldd<kCO>(dold + c[u]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 18; u++) {
+                        double d = (double)zj[u];
+                        if (!first) d += beta * dj[u];
+                        s += v[u] * d;
+                    }
+                    k = nc;
+                }
                 for (; k + 9 <= nc; k += 9) {
                     int c[9];
                     double v[9];
@@ -1251,6 +1276,12 @@ __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
         }
         return;
     }
+    // Uzawa iterations: every vector here is read only by the thread that
+    // writes it (element and equation loops use the same grid stride),
+    // except s2, which G s2 gathers across workgroups (agent-scope accesses)
+    // and Ahv, which the inner solve reads on its own rows (a fenced barrier);
+    // the inner solve ends with a fenced barrier, so u1 is read plainly.  The
+    // other waits need no cache maintenance.
     for (int e = gtid; e < nel; e += gsize) r1[e] = gt_elem(L, e, U.d);
     double dv = 1.0, dp = 1.0;
     int cycle = 0;
@@ -1266,7 +1297,7 @@ __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
             }
             reduce_publish<2>(U.rb, 4, acc, lds);
         }
-        ok = ok && grid_sync(U.gs, gen);
+        ok = csync(U.gs, gen, ok);
         double r1z1, delta = 0.0;
         {
             double o[2];
@@ -1274,13 +1305,14 @@ __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
             r1z1 = o[0];
             if (cycle > 0) delta = r1z1 / o[1];
         }
-        for (int e = gtid; e < nel; e += gsize) s2[e] = cycle == 0 ? z1[e] : z1[e] + delta * s1[e];
-        ok = ok && grid_sync(U.gs, gen);
+        for (int e = gtid; e < nel; e += gsize)
+            vstd<kCO>(s2 + e, cycle == 0 ? z1[e] : z1[e] + delta * vldd<kCO>(s1 + e));
+        ok = csync(U.gs, gen, ok);
         for (int i = gtid; i < neq; i += gsize) {
-            U.Ahv[i] = g_row(L, i, s2);
+            U.Ahv[i] = g_row_ld(L, i, [&](int e) { return vldd<kCO>(s2 + e); });
             U.u1[i] = 0.0;
         }
-        ok = ok && grid_sync(U.gs, gen);
+        ok = ok && grid_sync(U.gs, gen);   // the inner solve reads Ahv on its own rows
         MCL_PHASE(6);
         inner += inner_mixed(P, U.Ahv, U.u1, true, U.gs, gen, ok, lds, mgl, spmv, 0.0, nbr);
         {
@@ -1288,11 +1320,11 @@ __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
             for (int e = gtid; e < nel; e += gsize) {
                 double a = gt_elem(L, e, U.u1);
                 U.Ahp[e] = a;
-                acc[0] = dd_add_prod(acc[0], s2[e], a);
+                acc[0] = dd_add_prod(acc[0], vldd<kCO>(s2 + e), a);
             }
             reduce_publish<1>(U.rb, 6, acc, lds);
         }
-        ok = ok && grid_sync(U.gs, gen);
+        ok = csync(U.gs, gen, ok);
         double alpha;
         {
             double o[1];
@@ -1303,9 +1335,10 @@ __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
             dd acc[4] = {dd_zero(), dd_zero(), dd_zero(), dd_zero()};
             for (int e = gtid; e < nel; e += gsize) {
                 r2[e] = r1[e] - alpha * U.Ahp[e];
-                double pe = U.p[e] + alpha * s2[e];
+                const double se = vldd<kCO>(s2 + e);
+                double pe = U.p[e] + alpha * se;
                 U.p[e] = pe;
-                acc[0] = dd_add_prod(acc[0], s2[e], s2[e]);
+                acc[0] = dd_add_prod(acc[0], se, se);
                 acc[1] = dd_add_prod(acc[1], pe, pe);
             }
             for (int i = gtid; i < neq; i += gsize) {
@@ -1317,7 +1350,7 @@ __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
             }
             reduce_publish<4>(U.rb, 8, acc, lds);
         }
-        ok = ok && grid_sync(U.gs, gen);
+        ok = csync(U.gs, gen, ok);
         {
             double o[4];
             reduce_collect<4>(U.rb, 8, gridDim.x, o, lds);

@@ -30,15 +30,22 @@ __device__ __forceinline__ double ell_row(const int *__restrict__ col, const dou
 }
 
 // (G p)[P], G_prod_p (Solver.c:569-591): columns are the row's elements, ascending.
-__device__ __forceinline__ double g_row(const LevelDev &L, int P, const double *__restrict__ p)
+// (G p) for velocity row P with the element values read through load(e)
+template <typename LD>
+__device__ __forceinline__ double g_row_ld(const LevelDev &L, int P, LD load)
 {
     int n = L.veq_node[P], ip = L.veq_dof[P];
     double s = 0.0;
     for (int k = 0; k < L.p2n_cnt[n]; k++) {
         int e = L.p2n_el[k * L.nno + n], a = L.p2n_a[k * L.nno + n];
-        s += L.gtab[L.elev[e] * 8 + 2 * a + ip] * p[e];
+        s += L.gtab[L.elev[e] * 8 + 2 * a + ip] * load(e);
     }
     return s;
+}
+
+__device__ __forceinline__ double g_row(const LevelDev &L, int P, const double *__restrict__ p)
+{
+    return g_row_ld(L, P, [&](int e) { return p[e]; });
 }
 
 // (G^T x)[e], GT_prod_d (Solver.c:645-666): rows of the element ascending.
