@@ -62,7 +62,7 @@ struct LevelDev {
     const double *ch_w;
 };
 
-#define MCL_PHASES 12
+#define MCL_PHASES 16
 struct DevStats {
     int cycles, init_cycles, aborted, nan_flag;
     double dv, dp, dt;

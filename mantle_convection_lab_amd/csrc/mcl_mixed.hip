@@ -706,6 +706,16 @@ __device__ __forceinline__ bool csync(const GridSync &gs, unsigned &gen, bool ok
 // or for the whole grid when the lists are off.
 __device__ __forceinline__ bool vsync(const GridSync &gs, unsigned &gen, bool ok, int nbr, bool use_nbr)
 {
+#if MCL_WAIT_TIMERS
+    // split of the neighbour waits: work before the wait (12), the wait (13)
+    MCL_PHASE(12);
+    bool r;
+    if (!use_nbr) r = ok && grid_sync(gs, gen);
+    else if constexpr (kCO) r = ok && nbr_sync_coherent(gs, gen, nbr);
+    else r = ok && nbr_sync(gs, gen, nbr);
+    MCL_PHASE(13);
+    return r;
+#endif
     if (!use_nbr) return ok && grid_sync(gs, gen);
     if constexpr (kCO) return ok && nbr_sync_coherent(gs, gen, nbr);
     else return ok && nbr_sync(gs, gen, nbr);
