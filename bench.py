@@ -19,7 +19,8 @@ Paths (the Uzawa outer iteration and its stopping rule are the same in both):
                     [--path learned|mixed|fp64] [--no-cpu] [--copy-calibration]
 
 Prints one JSON line: value = converged Stokes solves per second of the primary
-path over its K timed steps (time steps 0..K-1 of a fresh instance after W
+path over its K timed steps (default 100: the workload's whole run; time steps
+0..K-1 of a fresh instance after W
 warmup steps on a throwaway instance; inputs already in HBM; wall clock between
 two device synchronisations), the plain FP64 path measured the same way in the
 same run over the same time steps, the roofline of the primary path's dominant kernel (the persistent Uzawa
@@ -244,7 +245,7 @@ def copy_calibration():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--fp64-steps", type=int, default=None,
                     help="timed steps of the plain FP64 comparison (default: --steps, after the same warmup)")
