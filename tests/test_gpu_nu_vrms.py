@@ -8,6 +8,8 @@ matches the reference to 1e-10 after one step) with its inner solves run to
 convergence (accuracy 1e-20), so that both paths solve the same discrete
 problem to the same Uzawa tolerance (dv, dp < 1e-6).  Bar: Vrms and Nu agree
 to 1e-6 relative after every step, and T to 1e-6 relative L2 at the end.
+The same bar holds over the first two steps of the 256x256 benchmark
+configuration.
 """
 import math
 
@@ -51,3 +53,33 @@ def test_nu_vrms_mixed_vs_fp64():
     assert worst_v < 1e-6
     assert worst_n < 1e-6
     assert et < 1e-6
+
+
+def test_nu_vrms_mixed_vs_fp64_256():
+    """The benchmark's own configuration (256x256, Ra=1e6, 1000x contrast,
+    seeded perturbation) over its first two time steps: the warm-started mixed
+    path against the FP64 path with converged inner solves (each FP64 step
+    takes about 10 s)."""
+    import bench
+    from mantle_convection_lab_amd import Input, Model
+    print("[gpu-nu-vrms] 256x256 benchmark configuration, 2 steps", flush=True)
+    ka = bench.path_keys("fp64")
+    ka.update(accuracy=1e-20, max_cg_cycle=100000)
+    a = Model(Input.from_keys(ka))
+    b = Model(Input.from_keys(bench.path_keys("mixed")))
+    for s in range(2):
+        ra = a.step()
+        rb = b.step()
+        assert ra["converged"] and rb["converged"]
+        va, na = a.diagnostics()
+        vb, nb = b.diagnostics()
+        ev, en = abs(vb - va) / abs(va), abs(nb - na) / abs(na)
+        print(f"  step {s}: fp64 uzawa={ra['uzawa']} mixed uzawa={rb['uzawa']}  Vrms {va:.12g} vs {vb:.12g} "
+              f"({ev:.1e})  Nu {na:.12g} vs {nb:.12g} ({en:.1e})", flush=True)
+        assert ev < 1e-6
+        assert en < 1e-6
+    et = rel_l2(b.field("T"), a.field("T"))
+    print(f"  T rel L2 {et:.2e}", flush=True)
+    assert et < 1e-6
+    a.ctx.close()
+    b.ctx.close()
