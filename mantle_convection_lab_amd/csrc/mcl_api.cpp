@@ -65,6 +65,8 @@ struct mcl_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     int nlev = 0;
+    const AsmLevel *asm_tab = nullptr;  // [nlev] all-levels assembly table (device)
+    int asm_blocks = 0;                 // workgroups of its launch
     std::vector<Level> levels;
     mcl_control ctl{};
     bool have_ctl = false;
@@ -817,6 +819,25 @@ extern "C" int mcl_mesh_upload(mcl_ctx *c, const mcl_mesh_view *levels, int nlev
         int rc = build_multigrid(c);
         if (rc) return rc;
     }
+    {
+        // all-levels assembly table (owned by the finest level, so a new
+        // upload frees it with the levels)
+        std::vector<AsmLevel> tab(nlev);
+        int blk = 0;
+        for (int l = 0; l < nlev; l++) {
+            Level &Lv = c->levels[l];
+            tab[l].L = Lv.dev;
+            tab[l].vis = Lv.vis;
+            tab[l].buoy = Lv.buoy;
+            tab[l].blk0 = blk;
+            blk += (Lv.neq + MCL_BLOCK - 1) / MCL_BLOCK;
+        }
+        const AsmLevel *tp = nullptr;
+        int rc = dev_upload(c, c->levels[nlev - 1].owned, &tp, tab);
+        if (rc) return rc;
+        c->asm_tab = tp;
+        c->asm_blocks = blk;
+    }
     // finest-level solver work space
     Level &F = c->levels[nlev - 1];
     for (void *p : {(void *)c->work, (void *)c->tau, (void *)c->hK, (void *)c->hC, (void *)c->hA, (void *)c->hF,
@@ -1074,10 +1095,9 @@ extern "C" int mcl_velocity_pressure_KGF_matrix(mcl_ctx *c)
     int rc = ready(c);
     if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->device));
-    for (auto &L : c->levels) {
-        if (L.neq == 0) continue;
-        hipLaunchKernelGGL(k_assemble_K, dim3(grid_for(c, L.neq)), dim3(MCL_BLOCK), 0, c->stream, L.dev,
-                           (const double *)L.vis, (const double *)L.buoy);
+    if (c->asm_blocks > 0) {
+        hipLaunchKernelGGL(k_assemble_K, dim3(c->asm_blocks), dim3(MCL_BLOCK), 0, c->stream,
+                           (const AsmLevel *)c->asm_tab, c->nlev);
         HIPCHK(c, hipGetLastError());
     }
     c->mixed_ready = false;

@@ -164,14 +164,33 @@ __device__ __forceinline__ void elem_eta(const LevelDev &L, int e, const double 
 // One thread per velocity row P: K row slots (element-ascending sums, the
 // order of K_matrix, Build_matrix.c:42-115), F[P] (all boundary-value
 // subtractions first, then the element loads of F_matrix, :117-140), Kdiag.
-// The thread owns its row, so the slots accumulate in place without atomics.
-__global__ void k_assemble_K(LevelDev L, const double *__restrict__ vis, const double *__restrict__ buoy)
+// The thread owns its row, so the slots accumulate without atomics: in the
+// thread's own LDS column for rows of at most 18 slots (written to HBM once at
+// the end), else in place in HBM; the additions to each slot come in the same
+// order either way.  One launch assembles every level: each workgroup takes
+// MCL_BLOCK rows of one level (lv[k].blk0 gives the first workgroup of level
+// k), so the short levels run side by side instead of one launch each.
+#define MCL_ASM_LDS_SLOTS 18
+__global__ void __launch_bounds__(MCL_BLOCK) k_assemble_K(const AsmLevel *__restrict__ lv, int nlev)
 {
-    for (int P = blockIdx.x * blockDim.x + threadIdx.x; P < L.neq; P += gridDim.x * blockDim.x) {
+    __shared__ double kacc[MCL_ASM_LDS_SLOTS * MCL_BLOCK];
+    double *acc = kacc + threadIdx.x;  // slot u at acc[u * MCL_BLOCK]
+    int lk = 0;
+    while (lk + 1 < nlev && (int)blockIdx.x >= lv[lk + 1].blk0) lk++;
+    const LevelDev &L = lv[lk].L;
+    const double *__restrict__ vis = lv[lk].vis;
+    const double *__restrict__ buoy = lv[lk].buoy;
+    const bool inl = L.WK <= MCL_ASM_LDS_SLOTS;
+    {
+        const int P = ((int)blockIdx.x - lv[lk].blk0) * MCL_BLOCK + (int)threadIdx.x;
+        if (P >= L.neq) return;
         const int n = L.veq_node[P], ip = L.veq_dof[P];
         const int nadj = L.p2n_cnt[n];
         const int nc = L.Kncol[P];
-        for (int s = 0; s < L.WK; s++) L.Kval[s * L.neq + P] = 0.0;
+        if (inl)
+            for (int u = 0; u < L.WK; u++) acc[u * MCL_BLOCK] = 0.0;
+        else
+            for (int s = 0; s < L.WK; s++) L.Kval[s * L.neq + P] = 0.0;
         double F = 0.0;
         for (int k = 0; k < nadj; k++) {
             int e = L.p2n_el[k * L.nno + n], a = L.p2n_a[k * L.nno + n];
@@ -186,7 +205,10 @@ __global__ void k_assemble_K(LevelDev L, const double *__restrict__ vis, const d
                     int q = 2 * b + j;
                     double kv = ke_entry(p, q, eta_g, dxdxi, dzdeta, jac);
                     int slot = L.Kslot[(k * 8 + q) * L.neq + P];
-                    if (slot >= 0) L.Kval[slot * L.neq + P] += kv;
+                    if (slot >= 0) {
+                        if (inl) acc[slot * MCL_BLOCK] += kv;
+                        else L.Kval[slot * L.neq + P] += kv;
+                    }
                     int nb = L.ienl[b * L.nel + e];
                     if (L.v_bc[j * L.nno + nb] == 'V') F -= kv * L.v_bc_val[j * L.nno + nb];
                 }
@@ -228,8 +250,16 @@ __global__ void k_assemble_K(LevelDev L, const double *__restrict__ vis, const d
             F += Fe;
         }
         double diag = 0.0;
-        for (int s = 0; s < nc; s++)
-            if (L.Kcol[s * L.neq + P] == P) diag = L.Kval[s * L.neq + P];
+        if (inl) {
+            for (int u = 0; u < L.WK; u++) {
+                const double v = acc[u * MCL_BLOCK];
+                L.Kval[u * L.neq + P] = v;
+                if (u < nc && L.Kcol[u * L.neq + P] == P) diag = v;
+            }
+        } else {
+            for (int s = 0; s < nc; s++)
+                if (L.Kcol[s * L.neq + P] == P) diag = L.Kval[s * L.neq + P];
+        }
         L.F[P] = F;
         L.Kdiag[P] = diag;
     }

@@ -200,7 +200,14 @@ __global__ void k_buoy_terms(int nent, int nno, const int *ent_e, const signed c
 __global__ void k_buoy_rows(int gnoz, int gnox, const int *ptr, const double *terms, double *avg);
 __global__ void k_buoy_apply(int nno, int gnoz, const int *node, const double *T, double Ra, const double *avg,
                              double *buoy);
-__global__ void k_assemble_K(LevelDev L, const double *vis, const double *buoy);
+// One level of the all-levels assembly launch: its rows occupy the
+// workgroups [blk0, blk0 + ceil(neq / MCL_BLOCK)) of the grid.
+struct AsmLevel {
+    LevelDev L;
+    const double *vis, *buoy;
+    int blk0;
+};
+__global__ void k_assemble_K(const AsmLevel *lv, int nlev);
 __global__ void k_uzawa_fp64(UzawaParams P);
 __global__ void k_v_to_nodes(LevelDev L, const double *d, double *V);
 __global__ void k_stress_elem(LevelDev L, const double *vis, const double *V, double *es, double *area);
