@@ -67,6 +67,8 @@ struct mcl_ctx {
     int nlev = 0;
     const AsmLevel *asm_tab = nullptr;  // [nlev] all-levels assembly table (device)
     int asm_blocks = 0;                 // workgroups of its launch
+    const PrepLevel *prep_tab = nullptr;  // [prep_n] FP32 preparation table of the mixed solve (device)
+    int prep_n = 0, prep_blocks = 0;
     std::vector<Level> levels;
     mcl_control ctl{};
     bool have_ctl = false;
@@ -838,6 +840,32 @@ extern "C" int mcl_mesh_upload(mcl_ctx *c, const mcl_mesh_view *levels, int nlev
         c->asm_tab = tp;
         c->asm_blocks = blk;
     }
+    {
+        // FP32 copies and Gershgorin bounds of the mixed solve's levels (all
+        // multigrid levels, or the finest alone without multigrid; bound m
+        // in lam_bits[m])
+        const int nm = c->have_mg ? nlev : 1;
+        std::vector<PrepLevel> tab(nm);
+        int blk = 0;
+        for (int m = 0; m < nm; m++) {
+            Level &Lv = c->have_mg ? c->levels[m] : c->levels.back();
+            tab[m].n = Lv.neq;
+            tab[m].W = Lv.WK;
+            tab[m].val = Lv.dev.Kval;
+            tab[m].diag = Lv.dev.Kdiag;
+            tab[m].val32 = Lv.val32;
+            tab[m].dinv = Lv.dinv32;
+            tab[m].bound = c->lam_bits + m;
+            tab[m].blk0 = blk;
+            blk += (Lv.neq + MCL_BLOCK - 1) / MCL_BLOCK;
+        }
+        const PrepLevel *tp = nullptr;
+        int rc = dev_upload(c, c->levels[nlev - 1].owned, &tp, tab);
+        if (rc) return rc;
+        c->prep_tab = tp;
+        c->prep_n = nm;
+        c->prep_blocks = blk;
+    }
     // finest-level solver work space
     Level &F = c->levels[nlev - 1];
     for (void *p : {(void *)c->work, (void *)c->tau, (void *)c->hK, (void *)c->hC, (void *)c->hA, (void *)c->hF,
@@ -1176,15 +1204,10 @@ static UzawaParams uzawa_params(mcl_ctx *c, double accuracy)
 // inverse of the coarsest K: once per assembly (mixed precision).
 static int mixed_prepare(mcl_ctx *c)
 {
-    const int nm = c->have_mg ? c->nlev : 1;
     HIPCHK(c, hipMemsetAsync(c->lam_bits, 0, MCL_MAX_LEVELS * sizeof(unsigned), c->stream));
-    for (int m = 0; m < nm; m++) {
-        Level &Lv = c->have_mg ? c->levels[m] : c->levels.back();
-        if (Lv.neq == 0) continue;
-        hipLaunchKernelGGL(k_to_fp32, dim3(grid_for(c, Lv.neq)), dim3(MCL_BLOCK), 0, c->stream, Lv.neq, Lv.WK,
-                           (const double *)Lv.dev.Kval, (const double *)Lv.dev.Kdiag, Lv.val32, Lv.dinv32);
-        hipLaunchKernelGGL(k_gershgorin, dim3(grid_for(c, Lv.neq)), dim3(MCL_BLOCK), 0, c->stream, Lv.neq, Lv.WK,
-                           (const double *)Lv.dev.Kval, (const double *)Lv.dev.Kdiag, c->lam_bits + m);
+    if (c->prep_blocks > 0) {
+        hipLaunchKernelGGL(k_prepare_fp32, dim3(c->prep_blocks), dim3(MCL_BLOCK), 0, c->stream,
+                           (const PrepLevel *)c->prep_tab, c->prep_n);
         HIPCHK(c, hipGetLastError());
     }
     if (c->have_mg) {

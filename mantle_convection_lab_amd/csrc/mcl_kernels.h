@@ -217,7 +217,15 @@ __global__ void k_collect3(RedBuf rb, int slot0, int nblocks, double *out);
 __global__ void k_heat_fp64(HeatParams H);
 __global__ void k_uzawa_mixed(MixedParams P);
 __global__ void k_uzawa_probe(MixedParams P);
-__global__ void k_to_fp32(int n, int W, const double *val, const double *diag, float *val32, float *dinv);
+// One multigrid level of the all-levels FP32 preparation launch: its rows
+// occupy the workgroups [blk0, blk0 + ceil(n / MCL_BLOCK)).
+struct PrepLevel {
+    int n, W;
+    const double *val, *diag;
+    float *val32, *dinv;
+    unsigned *bound;
+    int blk0;
+};
+__global__ void k_prepare_fp32(const PrepLevel *lv, int nlev);
 __global__ void k_coarse_inverse(int n, int W, const int *col, const int *ncol, const double *val, double *inv);
 __global__ void k_inject(int nc, const int *c2f, const double *fine, double *coarse);
-__global__ void k_gershgorin(int n, int W, const double *val, const double *diag, unsigned *bound_bits);

@@ -40,27 +40,28 @@ __shared__ long long mcl_ph[MCL_PHASES + 1];
 
 // FP32 copy of an FP64 ELL matrix, 1/diag, and the Gershgorin bound of D^-1 K
 // (max over rows of sum_j |K_ij| / K_ii), written as float bits with atomicMax
-// (positive floats order like their bit patterns).
-__global__ void k_to_fp32(int n, int W, const double *__restrict__ val, const double *__restrict__ diag,
-                          float *__restrict__ val32, float *__restrict__ dinv)
+// (positive floats order like their bit patterns).  One launch covers every
+// multigrid level: each workgroup takes MCL_BLOCK rows of one level.
+__global__ void __launch_bounds__(MCL_BLOCK) k_prepare_fp32(const PrepLevel *__restrict__ lv, int nlev)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        for (int s = 0; s < W; s++) val32[s * n + i] = (float)val[s * n + i];
-        dinv[i] = (float)(1.0 / diag[i]);
-    }
-}
-
-__global__ void k_gershgorin(int n, int W, const double *__restrict__ val, const double *__restrict__ diag,
-                             unsigned *__restrict__ bound_bits)
-{
+    int lk = 0;
+    while (lk + 1 < nlev && (int)blockIdx.x >= lv[lk + 1].blk0) lk++;
+    const PrepLevel &A = lv[lk];
+    const int i = ((int)blockIdx.x - A.blk0) * MCL_BLOCK + (int)threadIdx.x;
     float m = 0.f;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (i < A.n) {
         double s = 0.0;
-        for (int k = 0; k < W; k++) s += fabs(val[k * n + i]);
-        m = fmaxf(m, (float)(s / diag[i]));
+        for (int k = 0; k < A.W; k++) {
+            const double v = A.val[k * A.n + i];
+            A.val32[k * A.n + i] = (float)v;
+            s += fabs(v);
+        }
+        const double d = A.diag[i];
+        A.dinv[i] = (float)(1.0 / d);
+        m = (float)(s / d);
     }
     for (int o = MCL_WAVE / 2; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, MCL_WAVE));
-    if ((threadIdx.x & (MCL_WAVE - 1)) == 0) atomicMax(bound_bits, __float_as_uint(m));
+    if ((threadIdx.x & (MCL_WAVE - 1)) == 0) atomicMax(A.bound, __float_as_uint(m));
 }
 
 // Dense inverse of the coarsest K by in-place Gauss-Jordan elimination (K is
