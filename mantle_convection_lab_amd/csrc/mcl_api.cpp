@@ -634,9 +634,11 @@ static int block_rows_threshold()
     return e ? atoi(e) : 1024;
 }
 
-static unsigned coop_blocks(mcl_ctx *c, long long n, int rows_per_thread);
+static unsigned coop_blocks(mcl_ctx *c, long long n, int rows_per_block);
 static int build_neighbours(mcl_ctx *c, int nb);
-#define MCL_MIXED_ROWS_PER_THREAD 1
+// the mixed solve: one workgroup per 128 rows up to one per CU (256 from
+// 128x128 up; 64 at 64x64, 16 at 32x32), the fastest counts measured there
+#define MCL_MIXED_ROWS_PER_BLOCK 128
 
 static int build_multigrid(mcl_ctx *c)
 {
@@ -792,7 +794,7 @@ static int build_multigrid(mcl_ctx *c)
     // MCL_COOP_BLOCKS, rebuilds them at the first solve)
     if (maps) {
         Level &T = c->levels.back();
-        rc = build_neighbours(c, (int)coop_blocks(c, std::max(T.neq, T.nel), MCL_MIXED_ROWS_PER_THREAD));
+        rc = build_neighbours(c, (int)coop_blocks(c, std::max(T.neq, T.nel), MCL_MIXED_ROWS_PER_BLOCK));
         if (rc) return rc;
     }
     return 0;
@@ -1133,16 +1135,17 @@ extern "C" int mcl_velocity_pressure_KGF_matrix(mcl_ctx *c)
     return 0;
 }
 
-// Workgroups of a persistent solve: one per rows_per_thread * MCL_BLOCK rows,
-// at most what fits co-resident; the variable MCL_COOP_BLOCKS, if set,
-// replaces the count (still capped at co-residency).  The FP64 solvers take
-// two rows per thread (their phases are grid barriers, whose cost grows with
-// the workgroup count); the mixed solve takes one (its V-cycle waits are
-// mostly point-to-point, and more workgroups keep more loads in flight).
+// Workgroups of a persistent solve: one per rows_per_block rows, at most what
+// fits co-resident; the variable MCL_COOP_BLOCKS, if set, replaces the count
+// (still capped at co-residency).  The FP64 solvers take two rows per thread
+// (their phases are grid barriers, whose cost grows with the workgroup
+// count); the mixed solve takes at most 128 rows per workgroup (its V-cycle
+// waits are mostly point-to-point, and more workgroups keep more loads in
+// flight: at 256x256 and above every CU has one workgroup).
 #define MCL_FP64_ROWS_PER_THREAD 2
-static unsigned coop_blocks(mcl_ctx *c, long long n, int rows_per_thread)
+static unsigned coop_blocks(mcl_ctx *c, long long n, int rows_per_block)
 {
-    const long long per = (long long)rows_per_thread * MCL_BLOCK;
+    const long long per = rows_per_block;
     long long b = (n + per - 1) / per;
     const char *e = getenv("MCL_COOP_BLOCKS");
     if (e && atoi(e) > 0) b = atoi(e);
@@ -1196,7 +1199,7 @@ static UzawaParams uzawa_params(mcl_ctx *c, double accuracy)
     P.check_v_only = k.check_v_only;
     P.schur = k.schur;
     P.exact_dots = k.exact_dots;
-    P.gs.nblocks = coop_blocks(c, std::max(L.neq, L.nel), MCL_FP64_ROWS_PER_THREAD);
+    P.gs.nblocks = coop_blocks(c, std::max(L.neq, L.nel), MCL_FP64_ROWS_PER_THREAD * MCL_BLOCK);
     return P;
 }
 
@@ -1543,7 +1546,7 @@ static MixedParams mixed_params(mcl_ctx *c, const UzawaParams &P)
     const mcl_control &k = c->ctl;
     MixedParams M{};
     M.U = P;
-    M.U.gs.nblocks = coop_blocks(c, std::max(P.L.neq, P.L.nel), MCL_MIXED_ROWS_PER_THREAD);
+    M.U.gs.nblocks = coop_blocks(c, std::max(P.L.neq, P.L.nel), MCL_MIXED_ROWS_PER_BLOCK);
     M.mg = c->mg_dev;
     M.lam = (const float *)c->lam_bits;
     M.nmg = c->have_mg ? c->nlev : 1;
@@ -1712,7 +1715,7 @@ extern "C" int mcl_update_temperature(mcl_ctx *c, mcl_heat_stats *out)
     H.first_call = c->first_heat ? 1 : 0;
     H.cg_precondition = k.cg_precondition;
     H.exact_dots = k.exact_dots;
-    H.gs.nblocks = coop_blocks(c, std::max(L.neqT, L.nel), MCL_FP64_ROWS_PER_THREAD);
+    H.gs.nblocks = coop_blocks(c, std::max(L.neqT, L.nel), MCL_FP64_ROWS_PER_THREAD * MCL_BLOCK);
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));

@@ -13,9 +13,7 @@ the summed device time of the solves.  The best candidate per side is written
 to weights/tuning.json; `apply(side)` exports it before a context is created
 (variables the caller has set itself are left alone).  On the MI355X the
 tuner picks 256 workgroups, two-sweep fusion and two sweeps at 128x128,
-256x256 and 512x512: the library's defaults at 256x256 and 512x512, while at
-128x128 the default launches one workgroup per 512 rows (64), so there
-`apply(128)` changes the launch.
+256x256 and 512x512, the library's defaults there.
 
     python -m mantle_convection_lab_amd.tune [--sides 128 256 512] [--steps 3]
 """
