@@ -16,8 +16,7 @@ T and V within 1e-6 and p within 1e-3 relative L2 at every snapshot (the
 stopping rule bounds the pressure update, dp < 1e-6, not its error, which
 converges slowly in the near-checkerboard modes, so a last-ulp difference
 shows in p at the 1e-4 level); Vrms and
-the top Nusselt number of the last step within 1e-6 relative (the north-star
-bar).  (With the default compensated parallel sums the run follows the
+the top Nusselt number of the last step within 1e-6 relative.  (With the default compensated parallel sums the run follows the
 reference to 1e-8 in T until the capped solves, then separates at the level
 of the Uzawa tolerance: Vrms 1e-5, Nu 5e-5 after 500 steps.)
 """
