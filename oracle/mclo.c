@@ -496,7 +496,7 @@ static int conj_grad(const mclo_mesh *m, const vrows *A, const double *b, double
 }
 
 /* jacobi, Solver.c:1800-1864 (one rank): x_i = (b_i - sum_{j!=i} A_ij x_j^old)/A_ii,
- * stop when max|x - x_old| < accu. */
+ * stop when max|x - x_old| < accu.  [golden: jc_16] */
 static int jacobi_solve(const vrows *A, const double *b, double *x, double accu, int max_cycle,
                         long long *spmv)
 {
@@ -526,7 +526,8 @@ static int jacobi_solve(const vrows *A, const double *b, double *x, double accu,
     return cycle;
 }
 
-/* gauss_seidel, Solver.c:1866-1928: as jacobi but using updated x in place. */
+/* gauss_seidel, Solver.c:1866-1928: as jacobi but using updated x in place.
+ * [golden: gs_16, and as the smoother of mg_16 / mg_32] */
 static int gs_solve(const vrows *A, const double *b, double *x, double accu, int max_cycle,
                     long long *spmv)
 {
@@ -559,7 +560,10 @@ static int gs_solve(const vrows *A, const double *b, double *x, double accu, int
 /* Multigrid V-cycle as a stand-alone inner solver, Solver.c:668-769 at one
  * rank: Gauss-Seidel pre/post smoothing, injection restriction (project_res,
  * :770-832), averaging prolongation (interp_error, :1009-1143), coarse solve by
- * Gauss-Seidel (1000 sweeps); cycles until max|x - x_old| < accu. */
+ * Gauss-Seidel (1000 sweeps); cycles until max|x - x_old| < accu.  Below the
+ * finest level x aliases err[lev], so the prolongation overwrites the smoothed
+ * coarse iterate before it is added (x = 2 P e there), as in the reference's
+ * MG(), where x is G.err[lev] too.  [golden: mg_16, mg_32] */
 static void mg_vcycle(const mclo_ctrl *c, mclo_mg_level *mg, int lev, const double *rhs, double *x,
                       double **err, long long *spmv)
 {
@@ -635,7 +639,9 @@ static int inner_solve(const mclo_mesh *m, const mclo_ctrl *c, const vrows *A, c
 /* citcom_solver, Solver.c:14-249: Uzawa pressure-CG on the Schur complement
  * G^T K^-1 G with the diagonal BAB = sum_i G_ie^2 / Kdiag_i, applied as
  * z = BAB * r (reference) or z = r / BAB (schur_divide); stop when the update
- * ratios dv, dp fall below citcom_accu or at the cycle cap.  [golden] */
+ * ratios dv, dp fall below citcom_accu or at the cycle cap.  [golden; auto_accuracy:
+ * golden auto_16; schur_divide (not in the reference): closed, its converged solve
+ * equals a direct sparse solve of the saddle-point system] */
 int mclo_stokes_solve(const mclo_mesh *m, const mclo_ctrl *c,
                       const int *Kncol, const int *Kcol, const double *Kval, const double *Kdiag,
                       const int *Gncol, const int *Gcol, const double *Gval, const double *F,

@@ -64,12 +64,13 @@ typedef struct mclo_mesh {
 /* Geometric multigrid transfer maps between level l (coarse) and l+1 (fine),
  * the local parts of Connection.c:708-1500 at one rank:
  *   interp_n[u], interp_eq[k*neq_fine+u]   fine eq u <- mean of interp_n[u] coarse eqs (-1 = fixed)
- *   proj_l[k], proj_u[k]                   coarse eq proj_l[k] <- fine eq proj_u[k] (injection) */
+ *   proj_l[k], proj_u[k]                   coarse eq proj_l[k] <- fine eq proj_u[k] (injection),
+ *                                          held by the coarse level */
 typedef struct mclo_mg_level {
     const mclo_mesh *mesh;
     int *Kncol; int *Kcol; double *Kval; double *Kdiag;    /* assembled per level */
     const int *interp_n; const int *interp_eq;             /* maps to this level from the one below */
-    int nproj; const int *proj_l; const int *proj_u;       /* maps from this level to the one below */
+    int nproj; const int *proj_l; const int *proj_u;       /* injection into this level from the one above */
 } mclo_mg_level;
 
 typedef struct mclo_ctrl {

@@ -68,6 +68,21 @@ CASES = {
     # multigrid level maps (4 levels), capped solve
     "mg_16": (dict(level=5, mg_min_level=1, Rayleigh=1e4, T_ini=0, solver='"mg"', mg_cycle=2,
                    max_timestep=0, max_citcom_cycle=3), 1, 0, False),
+    # the reference's other inner solvers (Solver.c:82-89): Jacobi (its default
+    # "jc") and Gauss-Seidel, 1000 sweeps per inner solve, two full steps
+    "jc_16": (dict(level=5, mg_min_level=4, Rayleigh=1e4, T_ini=0, solver='"jc"', max_timestep=1,
+                   max_citcom_cycle=12), 2, 0, False),
+    "gs_16": (dict(level=5, mg_min_level=4, Rayleigh=1e4, T_ini=0, solver='"gs"', max_timestep=1,
+                   max_citcom_cycle=12), 2, 0, False),
+    # stand-alone multigrid with variable viscosity, three levels; the second
+    # step shows the coarse levels keeping the initial temperature (a T_ini=-1
+    # restart fills only the finest level, Initial_condition.c:171-195, so the
+    # analytic initial temperature is used)
+    "mg_32": (dict(level=6, mg_min_level=3, Rayleigh=1e5, A=2.302585092994046, refT=0.5, T_ini=0,
+                   solver='"mg"', mg_cycle=2, max_timestep=1, max_citcom_cycle=8), 2, 0, False),
+    # auto_accuracy (Solver.c:41-51): inner tolerance from |F|
+    "auto_16": (dict(level=5, mg_min_level=4, Rayleigh=1e5, T_ini=0, auto_accuracy=1, accuracy=1e-3,
+                     max_timestep=0, max_citcom_cycle=30), 1, 0, False),
     # 256x256 setup only (integer structures); solves cut to one iteration
     "setup_256": (dict(level=9, mg_min_level=8, Rayleigh=1e6, T_ini=0, max_timestep=0,
                        max_citcom_cycle=1, max_cg_cycle=1), 0, 0, False),

@@ -150,6 +150,41 @@ def mesh_from_fixture(fx, lev=None):
     return MeshArrays(d)
 
 
+def mg_from_fixture(fx, ctrl, T_levels):
+    """The oracle's multigrid hierarchy for a fixture with solver="mg": every
+    level min_level..max_level-1 assembled from its own temperature
+    T_levels[lev] (the reference assembles K on every level from that level's
+    T, Build_matrix.c:4-40), with the transfer maps the harness dumped
+    (interp[lev+1] and project[lev] under the coarse level's name).
+    Returns (levels, ctypes array of MgLevel, count, assembled systems)."""
+    L = int(fx["max_level"][0])
+    lo = int(fx["min_level"][0])
+    keep, systems = [], {}
+    arr = (MgLevel * (L - lo))()
+    for i, lev in enumerate(range(lo, L)):
+        m = mesh_from_fixture(fx, lev)
+        o = Oracle(m, ctrl)
+        T = np.ascontiguousarray(T_levels[lev], np.float64)
+        A = o.assemble(o.viscosity(T), o.buoyancy(T))
+        systems[lev] = A
+        keep += [m, A]
+        g = arr[i]
+        g.mesh = C.pointer(m.struct)
+        g.Kncol, g.Kcol, g.Kval, g.Kdiag = (_p(A["Kncol"], PI), _p(A["Kcol"], PI), _p(A["Kval"], PD),
+                                            _p(A["Kdiag"], PD))
+        if lev > lo:
+            n = np.ascontiguousarray(fx[f"L{lev - 1}.interp.local_eq_n"], np.int32)
+            eq = np.ascontiguousarray(fx[f"L{lev - 1}.interp.local_eq"], np.int32)
+            keep += [n, eq]
+            g.interp_n, g.interp_eq = _p(n, PI), _p(eq, PI)
+        if lev < L - 1:
+            pl = np.ascontiguousarray(fx[f"L{lev}.project.local_l"], np.int32)
+            pu = np.ascontiguousarray(fx[f"L{lev}.project.local_u"], np.int32)
+            keep += [pl, pu]
+            g.nproj, g.proj_l, g.proj_u = pl.size, _p(pl, PI), _p(pu, PI)
+    return keep, arr, L - lo, systems
+
+
 def ctrl_from_keys(k, schur_divide=0):
     f = lambda n, dflt: float(k.get(n, dflt))  # noqa: E731
     i = lambda n, dflt: int(k.get(n, dflt))  # noqa: E731
