@@ -35,16 +35,20 @@ def run(cmd, quiet=False):
 
 
 def build_hip():
+    """Compiles the units side by side (one hipcc each), then links."""
+    from concurrent.futures import ThreadPoolExecutor
     out = os.path.join(PKG, "libmcl_hip.so")
-    objs = []
+    jobs = []
     for src, extra in HIP_UNITS:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
         obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
-        run([HIPCC] + HIP_FLAGS + extra + ["-c", path, "-o", obj])
-        objs.append(obj)
-    run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", out] + objs)
+        jobs.append(([HIPCC] + HIP_FLAGS + extra + ["-c", path, "-o", obj], obj))
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", "8"))))
+    with ThreadPoolExecutor(workers) as ex:
+        list(ex.map(lambda j: run(j[0]), jobs))
+    run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", out] + [o for _, o in jobs])
     return out
 
 

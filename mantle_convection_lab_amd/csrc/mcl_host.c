@@ -1170,7 +1170,7 @@ int mclh_write_output(const mclh_input *in, const mclh_level *l, int timestep, c
     for (i = 0; i < l->nno; i++) fprintf(fp, "%e\n", vis[i]);
     fclose(fp);
     OPEN("velocity");
-    for (i = 0; i < l->nno; i++) fprintf(fp, "%e %e\n", V[i], V[l->nno + i]);
+    for (i = 0; i < l->nno; i++) fprintf(fp, "%+e %+e\n", V[i], V[l->nno + i]);
     fclose(fp);
 #undef OPEN
     return 0;

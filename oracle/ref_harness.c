@@ -206,15 +206,16 @@ static void dump_heat(int step)
 
 int main(int argc, char **argv)
 {
-    int full_steps = 1, snap_every = 0;
+    int full_steps = 1, snap_every = 0, write_files = 0;
     char name[96];
     int lev;
     if (argc < 3) {
-        fprintf(stderr, "usage: %s input dump [full_steps] [snapshot_every]\n", argv[0]);
+        fprintf(stderr, "usage: %s input dump [full_steps] [snapshot_every] [write_output]\n", argv[0]);
         return 2;
     }
     if (argc > 3) full_steps = atoi(argv[3]);
     if (argc > 4) snap_every = atoi(argv[4]);
+    if (argc > 5) write_files = atoi(argv[5]);
 
     MPI_Init(&argc, &argv);
     MPI_Comm_rank(MPI_COMM_WORLD, &G.ip);
@@ -271,6 +272,9 @@ int main(int argc, char **argv)
         solve_velocity_pressure();
         fflush(stdout);
         if (full) dump_solution(G.timestep);
+        /* the reference's own output files, Main.c:62-63 */
+        if (write_files && G.timestep % G.save_spacing == 0)
+            output();
         update_temperature();
         fflush(stdout);
         if (full) dump_heat(G.timestep);

@@ -39,7 +39,7 @@ BASE_KEYS = dict(
     save_spacing=100000, data_dir='"."',
 )
 
-# name -> (keys, full_steps, snapshot_every, seeded_T)
+# name -> (keys, full_steps, snapshot_every, seeded_T[, extras])
 CASES = {
     # C1: 64x64, Ra=1e4, isoviscous, T=(1-z)+0.01 cos(pi x) sin(pi z); first steps
     "c1_64": (dict(level=7, mg_min_level=6, Rayleigh=1e4, T_ini=0, max_timestep=2), 2, 1, False),
@@ -83,6 +83,13 @@ CASES = {
     # auto_accuracy (Solver.c:41-51): inner tolerance from |F|
     "auto_16": (dict(level=5, mg_min_level=4, Rayleigh=1e5, T_ini=0, auto_accuracy=1, accuracy=1e-3,
                      max_timestep=0, max_citcom_cycle=30), 1, 0, False),
+    # the reference's output files (Output.c:31-156, written at Main.c:62-63)
+    # for steps 0 and 1, and a restart (T_ini=-1, Initial_condition.c:171-195)
+    # from the temperature file of step 1
+    "out_16": (dict(level=5, mg_min_level=4, Rayleigh=1e4, T_ini=0, max_timestep=1, save_spacing=1,
+                    max_citcom_cycle=10), 2, 0, False, dict(write_output=True)),
+    "restart_16": (dict(level=5, mg_min_level=4, Rayleigh=1e4, T_ini=-1, max_timestep=0, max_citcom_cycle=10),
+                   1, 0, False, dict(restart_from=("out_16", 1))),
     # 256x256 setup only (integer structures); solves cut to one iteration
     "setup_256": (dict(level=9, mg_min_level=8, Rayleigh=1e6, T_ini=0, max_timestep=0,
                        max_citcom_cycle=1, max_cg_cycle=1), 0, 0, False),
@@ -120,11 +127,25 @@ def parse_stdout(text):
     return uz, tcg
 
 
+OUTPUT_FILES = ("coord", "temperature", "stress", "viscosity", "velocity")
+
+
 def run_case(name):
     keys = dict(BASE_KEYS)
-    ck, full_steps, snap, seeded = CASES[name]
+    ck, full_steps, snap, seeded = CASES[name][:4]
+    extras = CASES[name][4] if len(CASES[name]) > 4 else {}
     keys.update(ck)
     with tempfile.TemporaryDirectory() as td:
+        if "restart_from" in extras:
+            src, step = extras["restart_from"]
+            with np.load(os.path.join(HERE, src + ".npz")) as z:
+                text = bytes(z[f"out.s{step}.temperature"])
+            tdir = os.path.join(td, "oldT", "%07d" % step)
+            os.makedirs(tdir)
+            with open(os.path.join(tdir, "temperature.0.%07d" % step), "wb") as f:
+                f.write(text)
+            keys["old_T_file"] = '"%s"' % os.path.join(td, "oldT")
+            keys["old_T_timestep"] = step
         if seeded:
             level = keys["level"]
             side = 2 ** (level - 1)
@@ -139,11 +160,19 @@ def run_case(name):
         inp = os.path.join(td, "case.input")
         write_input(inp, keys)
         dump = os.path.join(td, "dump.bin")
-        res = subprocess.run([HARNESS, inp, dump, str(full_steps), str(snap)], cwd=td,
+        res = subprocess.run([HARNESS, inp, dump, str(full_steps), str(snap),
+                              "1" if extras.get("write_output") else "0"], cwd=td,
                              capture_output=True, text=True, timeout=3600)
         if res.returncode != 0:
             raise RuntimeError(f"{name}: harness failed\n{res.stderr[-2000:]}")
         data = read_dump(dump)
+        if extras.get("write_output"):
+            # the files exactly as the reference wrote them (data_dir ".")
+            for step in range(int(keys["max_timestep"]) + 1):
+                for f in OUTPUT_FILES:
+                    path = os.path.join(td, "%07d" % step, "%s.0.%07d" % (f, step))
+                    with open(path, "rb") as fh:
+                        data[f"out.s{step}.{f}"] = np.frombuffer(fh.read(), dtype=np.uint8)
     uz, tcg = parse_stdout(res.stdout)
     data["stdout.uzawa_cycles"] = np.array([u[0] for u in uz], dtype=np.int32)
     data["stdout.uzawa_dv"] = np.array([u[1] for u in uz])
