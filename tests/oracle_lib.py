@@ -150,6 +150,28 @@ def mesh_from_fixture(fx, lev=None):
     return MeshArrays(d)
 
 
+def mesh_from_host_level(lv):
+    """MeshArrays over one level of the host layer's mesh (Model.mesh.level(k)),
+    for meshes that no fixture holds (the 256x256 benchmark mesh)."""
+    nh = int(lv["nh"])
+    return MeshArrays(dict(L=lv["L"], lev=lv["lev"], nel=lv["nel"], nno=lv["nno"], neq=lv["neq"],
+                           neqT=lv["neqT"], gnox=lv["gnox"], gnoz=lv["gnoz"], Mcode=lv["Mcode"], ien=lv["ien"],
+                           node=lv["node"], id=lv["id"], lm=lv["lm"], heat_id=lv["heat_id"],
+                           heat_lm=lv["heat_lm"], v_bc=lv["v_bc"], v_bc_val=lv["v_bc_val"], T_bc=lv["T_bc"],
+                           T_bc_val=lv["T_bc_val"], X=lv["X"], dx=lv["dx"], dz=lv["dz"], nh=nh,
+                           h_ni=lv["h_ni"] if nh else np.zeros(1, np.int32),
+                           h_ci=lv["h_ci"] if nh else np.zeros(2, np.int32)))
+
+
+def ctrl_from_control(c, schur_divide=None, max_citcom_cycle=None):
+    """Oracle controls from the library's mcl_control (Input.control())."""
+    return Ctrl(c.Ra, c.A, c.refT, c.lm_jump, c.accuracy, c.citcom_accu, c.theta, c.Q,
+                c.max_citcom_cycle if max_citcom_cycle is None else max_citcom_cycle, c.max_cg_cycle,
+                c.cg_precondition, c.check_v_only, c.solver, c.auto_accuracy,
+                c.schur if schur_divide is None else schur_divide, c.mg_cycle, c.mg_finest_iteration,
+                c.mg_coarse_iteration)
+
+
 def mg_from_fixture(fx, ctrl, T_levels):
     """The oracle's multigrid hierarchy for a fixture with solver="mg": every
     level min_level..max_level-1 assembled from its own temperature
