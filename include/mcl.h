@@ -55,7 +55,7 @@ enum mcl_schur {
 /* Arithmetic of the Stokes solve. */
 enum mcl_precision {
     MCL_PREC_FP64 = 0,       /* FP64 throughout, reference operation order */
-    MCL_PREC_MIXED = 1       /* FP32 inner solves inside FP64 iterative refinement */
+    MCL_PREC_MIXED = 1       /* FP64 inner CG preconditioned by an FP32 multigrid V-cycle */
 };
 
 /* Fields that can be moved between host and device (per quadtree level). */
@@ -207,6 +207,7 @@ int mcl_update_temperature(mcl_ctx *ctx, mcl_heat_stats *stats);        /* Updat
 
 /* Synchronise the context's stream. */
 int mcl_synchronize(mcl_ctx *ctx);
+
 
 /* ---------------------------------------------------------------------------
  * Learned initial guess (no counterpart in the reference: new in this library).

@@ -76,6 +76,7 @@ struct mcl_ctx {
     bool been_here_dp = false;
     int max_coop_blocks = 0;
     int num_cu = 0;
+    int wall_khz = 0;                // rate of the device wall clock (wall_clock64)
     // solver work space (finest level)
     GridSync gs{};
     RedBuf rb{};
@@ -198,6 +199,7 @@ extern "C" int mcl_create(int device, mcl_ctx **out)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return MCL_E_DEVICE; }
     c->num_cu = prop.multiProcessorCount;
+    if (hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess) c->wall_khz = 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return MCL_E_DEVICE; }
     ShapeTables st;
     shape_tables(&st);
@@ -634,7 +636,7 @@ static int block_rows_threshold()
     return e ? atoi(e) : 1024;
 }
 
-static unsigned coop_blocks(mcl_ctx *c, long long n, int rows_per_block);
+static unsigned coop_blocks(mcl_ctx *c, long long n, int rows_per_block, const char *knob);
 static int build_neighbours(mcl_ctx *c, int nb);
 // the mixed solve: one workgroup per 128 rows up to one per CU (256 from
 // 128x128 up; 64 at 64x64, 16 at 32x32), the fastest counts measured there
@@ -791,10 +793,11 @@ static int build_multigrid(mcl_ctx *c)
     c->mixed_ready = false;
     // the mixed solve's neighbour and halo lists for its default workgroup
     // count, built here with the rest of the mesh set-up (a different count,
-    // MCL_COOP_BLOCKS, rebuilds them at the first solve)
+    // MCL_MIXED_BLOCKS, rebuilds them at the first solve)
     if (maps) {
         Level &T = c->levels.back();
-        rc = build_neighbours(c, (int)coop_blocks(c, std::max(T.neq, T.nel), MCL_MIXED_ROWS_PER_BLOCK));
+        rc = build_neighbours(c, (int)coop_blocks(c, std::max(T.neq, T.nel), MCL_MIXED_ROWS_PER_BLOCK,
+                                                  "MCL_MIXED_BLOCKS"));
         if (rc) return rc;
     }
     return 0;
@@ -1136,26 +1139,59 @@ extern "C" int mcl_velocity_pressure_KGF_matrix(mcl_ctx *c)
 }
 
 // Workgroups of a persistent solve: one per rows_per_block rows, at most what
-// fits co-resident; the variable MCL_COOP_BLOCKS, if set, replaces the count
-// (still capped at co-residency).  The FP64 solvers take two rows per thread
+// fits co-resident; the environment variable `knob` (MCL_FP64_BLOCKS for the
+// FP64 Uzawa and heat kernels, MCL_MIXED_BLOCKS for the mixed solve), if set,
+// replaces the count (still capped at co-residency).  The FP64 solvers take two rows per thread
 // (their phases are grid barriers, whose cost grows with the workgroup
 // count); the mixed solve takes at most 128 rows per workgroup (its V-cycle
 // waits are mostly point-to-point, and more workgroups keep more loads in
 // flight: at 256x256 and above every CU has one workgroup).
 #define MCL_FP64_ROWS_PER_THREAD 2
-static unsigned coop_blocks(mcl_ctx *c, long long n, int rows_per_block)
+static unsigned coop_blocks(mcl_ctx *c, long long n, int rows_per_block, const char *knob)
 {
     const long long per = rows_per_block;
     long long b = (n + per - 1) / per;
-    const char *e = getenv("MCL_COOP_BLOCKS");
+    const char *e = getenv(knob);
     if (e && atoi(e) > 0) b = atoi(e);
     return (unsigned)std::max<long long>(1, std::min<long long>(b, c->max_coop_blocks));
 }
 
-static int reset_sync_if_aborted(mcl_ctx *c, const DevStats &st)
+// Device buffers a Stokes solve works in besides the mesh and the fields:
+// grid-barrier words, reduction partials, the kernel's statistics, the
+// residual norms and the Uzawa / CG work vectors (6 n_v + 9 n_p doubles).
+// The context's own (allocated at set-up), or carved from the static scratch
+// array by the allocation-free solve.
+struct SolveWork {
+    GridSync gs;
+    RedBuf rb;
+    DevStats *stats;
+    double *resid;
+    double *work;
+};
+
+static SolveWork ctx_work(mcl_ctx *c) { return SolveWork{c->gs, c->rb, c->stats, c->resid, c->work}; }
+
+// Zero `bytes` (a multiple of 4) on the context's stream: hipMemsetAsync for
+// the allocating entry points, the zero-fill kernel for the allocation-free one.
+static int zero_async(mcl_ctx *c, void *p, size_t bytes, bool by_kernel)
+{
+    if (!by_kernel) {
+        HIPCHK(c, hipMemsetAsync(p, 0, bytes, c->stream));
+        return 0;
+    }
+    const long long n = (long long)(bytes / 4);
+    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<long long>(std::max(1LL, (n + MCL_BLOCK - 1) / MCL_BLOCK),
+                                                                         4LL * c->num_cu)),
+                       dim3(MCL_BLOCK), 0, c->stream, (unsigned *)p, n);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+static int reset_sync_if_aborted(mcl_ctx *c, const DevStats &st, const SolveWork &W, bool by_kernel)
 {
     if (st.aborted) {
-        HIPCHK(c, hipMemsetAsync(c->gs.gen, 0, MCL_SYNC_WORDS * sizeof(unsigned), c->stream));
+        int rc = zero_async(c, W.gs.gen, MCL_SYNC_WORDS * sizeof(unsigned), by_kernel);
+        if (rc) return rc;
         HIPCHK(c, hipStreamSynchronize(c->stream));
         return fail(c, MCL_E_DEVICE, "grid barrier timed out");
     }
@@ -1163,19 +1199,19 @@ static int reset_sync_if_aborted(mcl_ctx *c, const DevStats &st)
 }
 
 // Uzawa kernel parameters for the finest level's (d, p) and the work space.
-static UzawaParams uzawa_params(mcl_ctx *c, double accuracy)
+static UzawaParams uzawa_params(mcl_ctx *c, double accuracy, const SolveWork &W)
 {
     Level &L = c->levels.back();
     const mcl_control &k = c->ctl;
     UzawaParams P{};
     P.L = L.dev;
-    P.gs = c->gs;
-    P.rb = c->rb;
-    P.stats = c->stats;
+    P.gs = W.gs;
+    P.rb = W.rb;
+    P.stats = W.stats;
     P.d = L.d;
     P.p = L.p;
     size_t nv = std::max(L.neq, 1), np = L.nel;
-    double *w = c->work;
+    double *w = W.work;
     P.u1 = w; w += nv;
     P.Ahv = w; w += nv;
     P.cg_r = w; w += nv;
@@ -1199,7 +1235,7 @@ static UzawaParams uzawa_params(mcl_ctx *c, double accuracy)
     P.check_v_only = k.check_v_only;
     P.schur = k.schur;
     P.exact_dots = k.exact_dots;
-    P.gs.nblocks = coop_blocks(c, std::max(L.neq, L.nel), MCL_FP64_ROWS_PER_THREAD * MCL_BLOCK);
+    P.gs.nblocks = coop_blocks(c, std::max(L.neq, L.nel), MCL_FP64_ROWS_PER_THREAD * MCL_BLOCK, "MCL_FP64_BLOCKS");
     return P;
 }
 
@@ -1506,6 +1542,8 @@ static int build_neighbours(mcl_ctx *c, int nb)
                 }
         }
     }
+    c->nbr_nb = -1;
+    c->nbr_ok = 0;
     int rc = build_halos(c, nb, add);
     if (rc) return rc;
     std::vector<int> ptr(nb + 1, 0), idx;
@@ -1522,6 +1560,10 @@ static int build_neighbours(mcl_ctx *c, int nb)
         ptr[b + 1] = (int)idx.size();
         maxdeg = std::max(maxdeg, ptr[b + 1] - ptr[b]);
     }
+    // the old lists go first: until the new ones are in place no solve may
+    // use either (nbr_nb = -1 matches no workgroup count)
+    c->nbr_nb = -1;
+    c->nbr_ok = 0;
     for (void *q : {(void *)c->nbr_ptr, (void *)c->nbr_idx}) {
         if (q) {
             (void)hipFree(q);
@@ -1541,12 +1583,12 @@ static int build_neighbours(mcl_ctx *c, int nb)
     return 0;
 }
 
-static MixedParams mixed_params(mcl_ctx *c, const UzawaParams &P)
+static int mixed_params(mcl_ctx *c, const UzawaParams &P, MixedParams &M)
 {
     const mcl_control &k = c->ctl;
-    MixedParams M{};
+    M = MixedParams{};
     M.U = P;
-    M.U.gs.nblocks = coop_blocks(c, std::max(P.L.neq, P.L.nel), MCL_MIXED_ROWS_PER_BLOCK);
+    M.U.gs.nblocks = coop_blocks(c, std::max(P.L.neq, P.L.nel), MCL_MIXED_ROWS_PER_BLOCK, "MCL_MIXED_BLOCKS");
     M.mg = c->mg_dev;
     M.lam = (const float *)c->lam_bits;
     M.nmg = c->have_mg ? c->nlev : 1;
@@ -1558,7 +1600,11 @@ static MixedParams mixed_params(mcl_ctx *c, const UzawaParams &P)
     M.probe = 0;
     M.use_nbr = 0;
     const int nb = (int)M.U.gs.nblocks;
-    if (c->have_mg && nb != c->nbr_nb && build_neighbours(c, nb) == 0) {
+    if (c->have_mg && nb != c->nbr_nb) {
+        // halo and neighbour lists for this workgroup count; on failure both
+        // are left marked unusable and the solve is not launched
+        int rc = build_neighbours(c, nb);
+        if (rc) return rc;
     }
     if (c->have_mg && c->nbr_nb == nb) {
         M.nbr_ptr = c->nbr_ptr;
@@ -1568,86 +1614,112 @@ static MixedParams mixed_params(mcl_ctx *c, const UzawaParams &P)
     const char *sw = getenv("MCL_MG_SWEEPS");
     M.pre = sw && atoi(sw) > 0 ? atoi(sw) : 2;
     M.omega = 4.0f / 3.0f;
-    return M;
+    return 0;
 }
 
-extern "C" int mcl_solve_velocity_pressure(mcl_ctx *c, mcl_stokes_stats *out)
+// Two HIP events around a launch, destroyed on every return path.
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    int create(mcl_ctx *c)
+    {
+        HIPCHK(c, hipEventCreate(&a));
+        HIPCHK(c, hipEventCreate(&b));
+        return 0;
+    }
+    ~EventPair()
+    {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+// solve_velocity_pressure (Solver.c:3-12) on work space W.  noalloc: the
+// allocation-free form (FP64 only; no events, device time from the kernel's
+// clock stamps; every buffer cleared by the zero-fill kernel).
+static int solve_run(mcl_ctx *c, mcl_stokes_stats *out, const SolveWork &W, bool noalloc)
 {
-    int rc = ready(c);
-    if (rc) return rc;
-    HIPCHK(c, hipSetDevice(c->device));
     Level &L = c->levels.back();
     const mcl_control &k = c->ctl;
+    int rc;
     // initial_d_p (Solver.c:251-278): d = 0 on every level; p = 0 unless a
     // previous call exists, AMR is off and zero_P is off.
     if (!k.warm_start) {
         for (auto &Lv : c->levels) {
-            HIPCHK(c, hipMemsetAsync(Lv.d, 0, std::max(Lv.neq, 1) * sizeof(double), c->stream));
-            if (!(c->been_here_dp && k.AMR == 0 && k.zero_P == 0))
-                HIPCHK(c, hipMemsetAsync(Lv.p, 0, Lv.nel * sizeof(double), c->stream));
+            rc = zero_async(c, Lv.d, std::max(Lv.neq, 1) * sizeof(double), noalloc);
+            if (rc) return rc;
+            if (!(c->been_here_dp && k.AMR == 0 && k.zero_P == 0)) {
+                rc = zero_async(c, Lv.p, Lv.nel * sizeof(double), noalloc);
+                if (rc) return rc;
+            }
         }
     }
     c->been_here_dp = true;
-    HIPCHK(c, hipMemsetAsync(c->stats, 0, sizeof(DevStats), c->stream));
+    if (noalloc) {
+        // barrier words (the kernel starts from generation 0) and statistics
+        rc = zero_async(c, W.gs.gen, MCL_SYNC_WORDS * sizeof(unsigned), true);
+        if (rc) return rc;
+    }
+    rc = zero_async(c, W.stats, sizeof(DevStats), noalloc);
+    if (rc) return rc;
     double accuracy = k.accuracy;
     if (k.auto_accuracy) {
         // Solver.c:41-51: accuracy = citcom_accu * sqrt(F.F / neq)
-        hipLaunchKernelGGL(k_residuals, dim3(grid_for(c, L.neq)), dim3(MCL_BLOCK), 0, c->stream, L.dev, c->rb,
+        hipLaunchKernelGGL(k_residuals, dim3(grid_for(c, L.neq)), dim3(MCL_BLOCK), 0, c->stream, L.dev, W.rb,
                            (const double *)L.d, (const double *)L.p);
-        hipLaunchKernelGGL(k_collect3, dim3(1), dim3(MCL_BLOCK), 0, c->stream, c->rb, 12,
-                           (int)grid_for(c, L.neq), c->resid);
+        hipLaunchKernelGGL(k_collect3, dim3(1), dim3(MCL_BLOCK), 0, c->stream, W.rb, 12,
+                           (int)grid_for(c, L.neq), W.resid);
         double r[3];
-        HIPCHK(c, hipMemcpyAsync(r, c->resid, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(r, W.resid, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         accuracy = k.citcom_accu * std::sqrt(r[2] / L.neq);
     }
-    UzawaParams P = uzawa_params(c, accuracy);
-    hipEvent_t e0, e1;
-    HIPCHK(c, hipEventCreate(&e0));
-    HIPCHK(c, hipEventCreate(&e1));
+    UzawaParams P = uzawa_params(c, accuracy, W);
+    EventPair ev;
+    if (!noalloc) {
+        rc = ev.create(c);
+        if (rc) return rc;
+    }
     if (k.precision == MCL_PREC_MIXED) {
         if (!c->mixed_ready) {
             rc = mixed_prepare(c);
             if (rc) return rc;
         }
-        MixedParams M = mixed_params(c, P);
+        MixedParams M;
+        rc = mixed_params(c, P, M);
+        if (rc) return rc;
         void *margs[] = {&M};
-        HIPCHK(c, hipEventRecord(e0, c->stream));
+        if (!noalloc) HIPCHK(c, hipEventRecord(ev.a, c->stream));
         HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_uzawa_mixed, dim3(M.U.gs.nblocks), dim3(MCL_BLOCK), margs,
                                              0, c->stream));
     } else {
         void *args[] = {&P};
-        HIPCHK(c, hipEventRecord(e0, c->stream));
+        if (!noalloc) HIPCHK(c, hipEventRecord(ev.a, c->stream));
         HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_uzawa_fp64, dim3(P.gs.nblocks), dim3(MCL_BLOCK), args,
                                              0, c->stream));
     }
-    HIPCHK(c, hipEventRecord(e1, c->stream));
+    if (!noalloc) HIPCHK(c, hipEventRecord(ev.b, c->stream));
     // nodal velocity and stress (Solver.c:8-9)
-    for (auto &Lv : c->levels) {
-        if (&Lv != &L) continue;
-        hipLaunchKernelGGL(k_v_to_nodes, dim3(grid_for(c, Lv.nno)), dim3(MCL_BLOCK), 0, c->stream, Lv.dev,
-                           (const double *)Lv.d, Lv.V);
-        hipLaunchKernelGGL(k_stress_elem, dim3(grid_for(c, Lv.nel)), dim3(MCL_BLOCK), 0, c->stream, Lv.dev,
-                           (const double *)Lv.vis, (const double *)Lv.V, Lv.es, Lv.area);
-        hipLaunchKernelGGL(k_stress_node, dim3(grid_for(c, Lv.nno)), dim3(MCL_BLOCK), 0, c->stream, Lv.dev,
-                           (const double *)Lv.es, (const double *)Lv.area, Lv.stress);
-        HIPCHK(c, hipGetLastError());
-    }
-    hipLaunchKernelGGL(k_residuals, dim3(grid_for(c, L.neq)), dim3(MCL_BLOCK), 0, c->stream, L.dev, c->rb,
+    hipLaunchKernelGGL(k_v_to_nodes, dim3(grid_for(c, L.nno)), dim3(MCL_BLOCK), 0, c->stream, L.dev,
+                       (const double *)L.d, L.V);
+    hipLaunchKernelGGL(k_stress_elem, dim3(grid_for(c, L.nel)), dim3(MCL_BLOCK), 0, c->stream, L.dev,
+                       (const double *)L.vis, (const double *)L.V, L.es, L.area);
+    hipLaunchKernelGGL(k_stress_node, dim3(grid_for(c, L.nno)), dim3(MCL_BLOCK), 0, c->stream, L.dev,
+                       (const double *)L.es, (const double *)L.area, L.stress);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(k_residuals, dim3(grid_for(c, L.neq)), dim3(MCL_BLOCK), 0, c->stream, L.dev, W.rb,
                        (const double *)L.d, (const double *)L.p);
-    hipLaunchKernelGGL(k_collect3, dim3(1), dim3(MCL_BLOCK), 0, c->stream, c->rb, 12, (int)grid_for(c, L.neq),
-                       c->resid);
+    hipLaunchKernelGGL(k_collect3, dim3(1), dim3(MCL_BLOCK), 0, c->stream, W.rb, 12, (int)grid_for(c, L.neq),
+                       W.resid);
     HIPCHK(c, hipGetLastError());
     DevStats st;
     double r[3];
-    HIPCHK(c, hipMemcpyAsync(&st, c->stats, sizeof st, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(r, c->resid, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&st, W.stats, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(r, W.resid, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    rc = reset_sync_if_aborted(c, st);
+    if (!noalloc) hipEventElapsedTime(&ms, ev.a, ev.b);
+    else if (c->wall_khz > 0) ms = (float)((double)(st.t_end - st.t_begin) / c->wall_khz);
+    rc = reset_sync_if_aborted(c, st, W, noalloc);
     if (rc) return rc;
 #if MCL_PHASE_TIMERS
     {
@@ -1673,6 +1745,14 @@ extern "C" int mcl_solve_velocity_pressure(mcl_ctx *c, mcl_stokes_stats *out)
     }
     if (st.nan_flag) return fail(c, MCL_E_NUMERIC, "NaN in the Stokes solve");
     return 0;
+}
+
+extern "C" int mcl_solve_velocity_pressure(mcl_ctx *c, mcl_stokes_stats *out)
+{
+    int rc = ready(c);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    return solve_run(c, out, ctx_work(c), false);
 }
 
 extern "C" int mcl_update_temperature(mcl_ctx *c, mcl_heat_stats *out)
@@ -1715,7 +1795,7 @@ extern "C" int mcl_update_temperature(mcl_ctx *c, mcl_heat_stats *out)
     H.first_call = c->first_heat ? 1 : 0;
     H.cg_precondition = k.cg_precondition;
     H.exact_dots = k.exact_dots;
-    H.gs.nblocks = coop_blocks(c, std::max(L.neqT, L.nel), MCL_FP64_ROWS_PER_THREAD * MCL_BLOCK);
+    H.gs.nblocks = coop_blocks(c, std::max(L.neqT, L.nel), MCL_FP64_ROWS_PER_THREAD * MCL_BLOCK, "MCL_FP64_BLOCKS");
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
@@ -1733,7 +1813,7 @@ extern "C" int mcl_update_temperature(mcl_ctx *c, mcl_heat_stats *out)
     hipEventElapsedTime(&ms, e0, e1);
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    rc = reset_sync_if_aborted(c, st);
+    rc = reset_sync_if_aborted(c, st, ctx_work(c), false);
     if (rc) return rc;
     c->first_heat = false;
     if (out) {
@@ -1939,10 +2019,12 @@ extern "C" int mcl_net_capture(mcl_ctx *c, mcl_net *net)
 static int uzawa_probe(mcl_ctx *c, double *d, double *p, double *r1)
 {
     HIPCHK(c, hipMemsetAsync(c->stats, 0, sizeof(DevStats), c->stream));
-    UzawaParams P = uzawa_params(c, c->ctl.accuracy);
+    UzawaParams P = uzawa_params(c, c->ctl.accuracy, ctx_work(c));
     P.d = d;
     P.p = p;
-    MixedParams M = mixed_params(c, P);
+    MixedParams M;
+    int rc = mixed_params(c, P, M);
+    if (rc) return rc;
     M.probe = 1;
     {
         // the probe only ranks two candidates: a looser inner solve suffices
@@ -1956,7 +2038,7 @@ static int uzawa_probe(mcl_ctx *c, double *d, double *p, double *r1)
     DevStats st;
     HIPCHK(c, hipMemcpyAsync(&st, c->stats, sizeof st, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    int rc = reset_sync_if_aborted(c, st);
+    rc = reset_sync_if_aborted(c, st, ctx_work(c), false);
     if (rc) return rc;
     *r1 = std::sqrt(st.probe_r1);
     return 0;

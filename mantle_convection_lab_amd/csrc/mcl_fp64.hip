@@ -410,6 +410,7 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_uzawa_fp64(UzawaParams P)
     const LevelDev &L = P.L;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsize = gridDim.x * blockDim.x;
     const int neq = L.neq, nel = L.nel;
+    if (blockIdx.x == 0 && threadIdx.x == 0) P.stats->t_begin = wall_clock64();
     unsigned gen = __hip_atomic_load(P.gs.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     bool ok = true;
     long long spmv = 0, inner = 0;
@@ -543,7 +544,16 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_uzawa_fp64(UzawaParams P)
         P.stats->spmv = spmv;
         P.stats->aborted = ok ? 0 : 1;
         if (!(dv == dv) || !(dp == dp)) P.stats->nan_flag = 1;
+        P.stats->t_end = wall_clock64();
     }
+}
+
+// Zero n 32-bit words (a kernel rather than hipMemsetAsync, so that a caller
+// may capture it in a graph).
+__global__ void k_zero_words(unsigned *p, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0u;
 }
 
 // v_to_nodes (Solver.c:1319-1343)

@@ -69,6 +69,7 @@ struct DevStats {
     long long inner, spmv;
     long long phase[MCL_PHASES];   // 100 MHz ticks per solver phase (builds with MCL_PHASE_TIMERS)
     double probe_r1;               // |G^T d|^2 after the first inner solve (probe launches)
+    long long t_begin, t_end;      // device wall clock (wall_clock64) at the FP64 Uzawa kernel's start and end
 };
 
 struct UzawaParams {
@@ -229,3 +230,4 @@ struct PrepLevel {
 __global__ void k_prepare_fp32(const PrepLevel *lv, int nlev);
 __global__ void k_coarse_inverse(int n, int W, const int *col, const int *ncol, const double *val, double *inv);
 __global__ void k_inject(int nc, const int *c2f, const double *fine, double *coarse);
+__global__ void k_zero_words(unsigned *p, long long n);
