@@ -208,6 +208,20 @@ int mcl_update_temperature(mcl_ctx *ctx, mcl_heat_stats *stats);        /* Updat
 /* Synchronise the context's stream. */
 int mcl_synchronize(mcl_ctx *ctx);
 
+/* mcl_solve_velocity_pressure without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * FP64 Uzawa solve (precision MCL_PREC_FP64; MCL_E_UNSUPPORTED otherwise) with
+ * the same kernels and results as mcl_solve_velocity_pressure, but its work
+ * space -- grid-barrier words, reduction partials, statistics, residual norms
+ * and the 6 n_v + 9 n_p work vectors -- lives in a static device array of
+ * 48 MiB per GPU (held until the process ends), located at mcl_create; no
+ * hipMalloc, hipFree or event is created, buffers are cleared by a kernel and
+ * stats->device_ms comes from the kernel's own clock.  Meshes whose work space
+ * exceeds the array (above about 512 x 512 elements) are rejected with
+ * MCL_E_ARG.  Every call on one GPU shares that GPU's array: calls that overlap
+ * on one GPU race. */
+int mcl_solve_velocity_pressure_noalloc(mcl_ctx *ctx, mcl_stokes_stats *stats);
 
 /* ---------------------------------------------------------------------------
  * Learned initial guess (no counterpart in the reference: new in this library).

@@ -127,6 +127,7 @@ def hip_lib():
         for n in ("mcl_buoyancy_force", "mcl_viscosity", "mcl_velocity_pressure_KGF_matrix", "mcl_synchronize"):
             getattr(L, n).argtypes = [vp]
         L.mcl_solve_velocity_pressure.argtypes = [vp, C.POINTER(StokesStats)]
+        L.mcl_solve_velocity_pressure_noalloc.argtypes = [vp, C.POINTER(StokesStats)]
         L.mcl_update_temperature.argtypes = [vp, C.POINTER(HeatStats)]
         _hip = L
     return _hip

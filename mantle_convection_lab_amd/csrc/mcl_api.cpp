@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -183,6 +184,34 @@ static void shape_tables(ShapeTables *s)
     s->detac[0] = -0.25; s->detac[1] = 0.25; s->detac[2] = 0.25; s->detac[3] = -0.25;
 }
 
+// ------------------------------------------------------------ static scratch
+// Work space of mcl_solve_velocity_pressure_noalloc: the FP64 Uzawa solve's
+// grid-barrier words, statistics, residual norms, reduction partials and work
+// vectors, in one file-scope device array per GPU (each GPU the library runs
+// on holds its copy until the process ends).  Host code never uses the array's
+// name as an address: each device's copy is located once, with that device
+// current, by hipGetSymbolAddress at mcl_create, and only that pointer is
+// handed to kernels.  Every call on a GPU shares its copy, so allocation-free
+// solves that overlap on one GPU race.
+#define MCL_SCRATCH_BYTES (48ull << 20)
+#define MCL_SCRATCH_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[MCL_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch[MCL_SCRATCH_DEVICES];
+static hipError_t g_scratch_err[MCL_SCRATCH_DEVICES];
+
+// Locate the current device's copy (once per device); call with `device` current.
+static void scratch_locate(int device)
+{
+    if (device < 0 || device >= MCL_SCRATCH_DEVICES) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = e;
+    if (e == hipSuccess) g_scratch[device] = (unsigned char *)p;
+}
+
 extern "C" int mcl_abi_version(void) { return MCL_ABI_VERSION; }
 
 extern "C" const char *mcl_last_error(const mcl_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -201,6 +230,7 @@ extern "C" int mcl_create(int device, mcl_ctx **out)
     c->num_cu = prop.multiProcessorCount;
     if (hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess) c->wall_khz = 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return MCL_E_DEVICE; }
+    scratch_locate(device);
     ShapeTables st;
     shape_tables(&st);
     if (mcl_upload_shape(&st) != hipSuccess) { delete c; return MCL_E_DEVICE; }
@@ -1753,6 +1783,59 @@ extern "C" int mcl_solve_velocity_pressure(mcl_ctx *c, mcl_stokes_stats *out)
     if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->device));
     return solve_run(c, out, ctx_work(c), false);
+}
+
+// Byte offsets of the allocation-free solve's buffers in the scratch array
+// (each 256-aligned); total = bytes needed.
+struct ScratchLayout {
+    size_t sync, stats, resid, part, pmax, work, total;
+};
+
+static ScratchLayout scratch_layout(size_t nv, size_t np)
+{
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    ScratchLayout S{};
+    size_t o = 0;
+    S.sync = o; o = up(o + MCL_SYNC_WORDS * sizeof(unsigned));
+    S.stats = o; o = up(o + sizeof(DevStats));
+    S.resid = o; o = up(o + 4 * sizeof(double));
+    S.part = o; o = up(o + (size_t)MCL_RED_SLOTS * MCL_MAX_WG * sizeof(dd));
+    S.pmax = o; o = up(o + (size_t)MCL_RED_SLOTS * MCL_MAX_WG * sizeof(double));
+    S.work = o; o = up(o + (6 * nv + 9 * np) * sizeof(double));
+    S.total = o;
+    return S;
+}
+
+extern "C" int mcl_solve_velocity_pressure_noalloc(mcl_ctx *c, mcl_stokes_stats *out)
+{
+    int rc = ready(c);
+    if (rc) return rc;
+    if (c->ctl.precision != MCL_PREC_FP64)
+        return fail(c, MCL_E_UNSUPPORTED, "the allocation-free solve runs the FP64 Uzawa kernel only (precision 0)");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->device < 0 || c->device >= MCL_SCRATCH_DEVICES || !g_scratch[c->device])
+        return fail(c, MCL_E_DEVICE, "static scratch of device %d not located: %s", c->device,
+                    hipGetErrorString(c->device >= 0 && c->device < MCL_SCRATCH_DEVICES ? g_scratch_err[c->device]
+                                                                                        : hipErrorInvalidDevice));
+    Level &L = c->levels.back();
+    const size_t nv = std::max(L.neq, 1), np = L.nel;
+    const ScratchLayout S = scratch_layout(nv, np);
+    if (S.total > MCL_SCRATCH_BYTES)
+        return fail(c, MCL_E_ARG,
+                    "mesh too large for the allocation-free solve: %zu velocity and %zu pressure unknowns need %zu "
+                    "bytes of scratch, the maximum is %llu",
+                    nv, np, S.total, (unsigned long long)MCL_SCRATCH_BYTES);
+    unsigned char *base = g_scratch[c->device];
+    SolveWork W{};
+    W.gs.gen = (unsigned *)(base + S.sync);
+    W.gs.abort = W.gs.gen + 32;
+    W.gs.arrive = W.gs.gen + 64;
+    W.stats = (DevStats *)(base + S.stats);
+    W.resid = (double *)(base + S.resid);
+    W.rb.part = (dd *)(base + S.part);
+    W.rb.pmax = (double *)(base + S.pmax);
+    W.work = (double *)(base + S.work);
+    return solve_run(c, out, W, true);
 }
 
 extern "C" int mcl_update_temperature(mcl_ctx *c, mcl_heat_stats *out)

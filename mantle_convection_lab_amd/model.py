@@ -262,6 +262,13 @@ class Context:
         self._ck(self.L.mcl_solve_velocity_pressure(self.h, C.byref(st)))
         return st
 
+    def solve_velocity_pressure_noalloc(self):
+        """The allocation-free form (work space in the library's static
+        device scratch array; FP64 precision only)."""
+        st = N.StokesStats()
+        self._ck(self.L.mcl_solve_velocity_pressure_noalloc(self.h, C.byref(st)))
+        return st
+
     def update_temperature(self):
         st = N.HeatStats()
         self._ck(self.L.mcl_update_temperature(self.h, C.byref(st)))
