@@ -147,11 +147,16 @@ static void slot_sort(int n, int *col, double *val)
 void mclo_rows_prod(int n, int W, const int *ncol, const int *col, const double *val,
                     const double *x, double *y)
 {
-    int i, j;
-    for (i = 0; i < n; i++) y[i] = 0.0;
-    for (i = 0; i < n; i++)
-        for (j = 0; j < ncol[i]; j++)
-            y[i] += val[(size_t)i * W + j] * x[col[(size_t)i * W + j]];
+    int i;
+    /* rows are independent and each keeps its sequential sum: the threads
+     * change nothing in the arithmetic */
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < n; i++) {
+        int j;
+        double s = 0.0;
+        for (j = 0; j < ncol[i]; j++) s += val[(size_t)i * W + j] * x[col[(size_t)i * W + j]];
+        y[i] = s;
+    }
 }
 
 /* ------------------------------------------------------------------- buoyancy
@@ -455,6 +460,7 @@ static int conj_grad(const mclo_mesh *m, const vrows *A, const double *b, double
     double *q = malloc(n * sizeof(double)), *s = malloc(n * sizeof(double));
     double delta_new, delta_old, delta0, alpha, beta, t1;
     kprod(A, x, r, spmv);
+#pragma omp parallel for schedule(static)
     for (i = 0; i < n; i++) {
         r[i] = b[i] - r[i];
         dd[i] = precond ? r[i] / A->diag[i] : r[i];
@@ -466,27 +472,35 @@ static int conj_grad(const mclo_mesh *m, const vrows *A, const double *b, double
         kprod(A, dd, q, spmv);
         t1 = vdot(n, dd, q);
         alpha = delta_new / t1;
+#pragma omp parallel for schedule(static)
         for (i = 0; i < n; i++) x[i] += alpha * dd[i];
         constrain_v(m, x);
         if (cycle % 50 == 0) {
             kprod(A, x, r, spmv);
+#pragma omp parallel for schedule(static)
             for (i = 0; i < n; i++) r[i] = b[i] - r[i];
         } else {
+#pragma omp parallel for schedule(static)
             for (i = 0; i < n; i++) r[i] -= alpha * q[i];
         }
         if (precond) {
+#pragma omp parallel for schedule(static)
             for (i = 0; i < n; i++) s[i] = r[i] / A->diag[i];
             delta_old = delta_new;
             delta_new = vdot(n, r, s);
             beta = delta_new / delta_old;
+#pragma omp parallel for schedule(static)
             for (i = 0; i < n; i++) dd[i] = s[i] + beta * dd[i];
         } else {
             delta_old = delta_new;
             delta_new = vdot(n, r, r);
             beta = delta_new / delta_old;
+#pragma omp parallel for schedule(static)
             for (i = 0; i < n; i++) dd[i] = r[i] + beta * dd[i];
         }
         cycle++;
+        /* a maximum does not depend on the order it is taken in */
+#pragma omp parallel for schedule(static) reduction(max : res)
         for (i = 0; i < n; i++)
             if (fabs(alpha * r[i]) > res) res = fabs(alpha * r[i]);
         if (res < accu) break;
