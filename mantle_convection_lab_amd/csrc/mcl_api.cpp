@@ -53,6 +53,12 @@ struct Level {
     float *rval = nullptr;
     int rw = 0;
     std::vector<int> rcol_h;                  // host copy of rcol ([rw][neq], -1 padded)
+    // injection into the level below (the reference's project maps, solver="mg")
+    std::vector<int> proj_l, proj_u;
+    // Jacobi / Gauss-Seidel / multigrid inner solvers: wavefront schedule and level vectors
+    int nsched = 0;
+    int *sptr = nullptr, *srows = nullptr, *dproj_l = nullptr, *dproj_u = nullptr;
+    double *gerr = nullptr, *grhs = nullptr, *gres = nullptr;
     // buoyancy row lists
     int *brow_ptr = nullptr, *brow_e = nullptr;
     signed char *brow_xi = nullptr, *brow_eta = nullptr;
@@ -99,6 +105,10 @@ struct mcl_ctx {
     int nbr_nb = 0;                  // workgroup count the neighbour lists were built for
     int nbr_ok = 0;                  // lists usable (every list within MCL_BLOCK)
     int *nbr_ptr = nullptr, *nbr_idx = nullptr;
+    // the reference's stationary inner solvers (jc, gs, mg)
+    const GsLevelDev *gs_tab = nullptr;  // [nlev] (device)
+    bool gs_mg_ok = false;           // transfer maps of every level present (solver="mg")
+    double *xt = nullptr, *xt2 = nullptr;
     double *gd = nullptr, *gp = nullptr;  // learned-guess candidate (d, p)
     // learned guess: per-row partial sums and the sample scale [2 H + 1]
     double *net_rows = nullptr;
@@ -440,6 +450,13 @@ static int build_level(mcl_ctx *c, const mcl_mesh_view &v, Level &Lv)
     Lv.WT = WT;
     Lv.Tncol_h = Tncol;
     Lv.Tcol_h = Tcol;
+    if (v.nproj > 0 && v.proj_l && v.proj_u) {
+        Lv.proj_l.assign(v.proj_l, v.proj_l + v.nproj);
+        Lv.proj_u.assign(v.proj_u, v.proj_u + v.nproj);
+        for (int k = 0; k < v.nproj; k++)
+            if (Lv.proj_u[k] < 0 || Lv.proj_u[k] >= neq)
+                return fail(c, MCL_E_ARG, "proj_u[%d] = %d outside the level's %d equations", k, Lv.proj_u[k], neq);
+    }
     if (v.interp_n && v.interp_eq) {
         Lv.interp_n.assign(v.interp_n, v.interp_n + neq);
         Lv.interp_eq.assign(v.interp_eq, v.interp_eq + 4 * (size_t)neq);
@@ -833,6 +850,113 @@ static int build_multigrid(mcl_ctx *c)
     return 0;
 }
 
+// Wavefront schedule of a Gauss-Seidel sweep over rows 0..n-1 in order: row i
+// must be updated after every lower-numbered row it is coupled to (it reads
+// their new values) and before every higher-numbered one (they read its old
+// value), so group(i) = 1 + max group of its coupled lower rows; rows of one
+// group are independent.  The coupling is taken in both directions, so an
+// unsymmetric pattern is handled too.
+static void gs_schedule(const Level &Lv, std::vector<int> &ptr, std::vector<int> &rows)
+{
+    const int n = Lv.neq;
+    std::vector<std::vector<int>> lower(n);
+    for (int i = 0; i < n; i++)
+        for (int s = 0; s < Lv.Kncol_h[i]; s++) {
+            const int j = Lv.Kcol_h[(size_t)s * n + i];
+            if (j < i) lower[i].push_back(j);
+            else if (j > i) lower[j].push_back(i);
+        }
+    std::vector<int> grp(n, 0);
+    int ng = 0;
+    for (int i = 0; i < n; i++) {
+        int g = 0;
+        for (int j : lower[i]) g = std::max(g, grp[j] + 1);
+        grp[i] = g;
+        ng = std::max(ng, g + 1);
+    }
+    ptr.assign(ng + 1, 0);
+    for (int i = 0; i < n; i++) ptr[grp[i] + 1]++;
+    for (int g = 0; g < ng; g++) ptr[g + 1] += ptr[g];
+    rows.assign(std::max(n, 1), 0);
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < n; i++) rows[fill[grp[i]]++] = i;   // ascending within a group
+}
+
+// Data of the reference's stationary inner solvers (solver = jc, gs, mg):
+// per level the wavefront schedule, the injection maps and three vectors,
+// the level table, and two finest-level vectors.
+static int build_stationary(mcl_ctx *c)
+{
+    const int nlev = c->nlev;
+    std::vector<GsLevelDev> tab(nlev);
+    bool mg_ok = nlev >= 2;
+    for (int k = 0; k < nlev; k++) {
+        Level &Lv = c->levels[k];
+        std::vector<int> ptr, rows;
+        gs_schedule(Lv, ptr, rows);
+        const int *cp;
+        if (dev_upload(c, Lv.owned, &cp, ptr)) return MCL_E_DEVICE;
+        Lv.sptr = (int *)cp;
+        if (dev_upload(c, Lv.owned, &cp, rows)) return MCL_E_DEVICE;
+        Lv.srows = (int *)cp;
+        Lv.nsched = (int)ptr.size() - 1;
+        const int n = std::max(Lv.neq, 1);
+        if (dev_alloc(c, Lv.owned, &Lv.gerr, n) || dev_alloc(c, Lv.owned, &Lv.grhs, n) ||
+            dev_alloc(c, Lv.owned, &Lv.gres, n))
+            return MCL_E_DEVICE;
+        GsLevelDev &G = tab[k];
+        G.n = Lv.neq;
+        G.WK = Lv.WK;
+        G.col = Lv.dev.Kcol;
+        G.ncol = Lv.dev.Kncol;
+        G.val = Lv.dev.Kval;
+        G.nsched = Lv.nsched;
+        G.sptr = Lv.sptr;
+        G.srows = Lv.srows;
+        G.err = Lv.gerr;
+        G.rhs = Lv.grhs;
+        G.resu = Lv.gres;
+        if (k > 0) {
+            Level &C = c->levels[k - 1];
+            // every equation of the level below must receive an injected residual
+            std::vector<char> hit(std::max(C.neq, 1), 0);
+            for (size_t q = 0; q < Lv.proj_l.size(); q++) {
+                const int l = Lv.proj_l[q];
+                if (l < 0 || l >= C.neq) return fail(c, MCL_E_ARG, "proj_l[%zu] = %d outside level %d", q, l, C.lev);
+                hit[l] = 1;
+            }
+            bool covered = !Lv.proj_l.empty();
+            for (int l = 0; l < C.neq && covered; l++) covered = hit[l] != 0;
+            if (!covered || Lv.interp_n.empty() || !Lv.pn) mg_ok = false;
+            G.pn = Lv.pn;
+            G.pcol = Lv.pcol;
+            G.nproj = (int)Lv.proj_l.size();
+            if (G.nproj > 0) {
+                if (dev_upload(c, Lv.owned, &cp, Lv.proj_l)) return MCL_E_DEVICE;
+                Lv.dproj_l = (int *)cp;
+                if (dev_upload(c, Lv.owned, &cp, Lv.proj_u)) return MCL_E_DEVICE;
+                Lv.dproj_u = (int *)cp;
+            }
+            G.proj_l = Lv.dproj_l;
+            G.proj_u = Lv.dproj_u;
+        }
+    }
+    const GsLevelDev *tp;
+    if (dev_upload(c, c->levels.back().owned, &tp, tab)) return MCL_E_DEVICE;
+    c->gs_tab = tp;
+    c->gs_mg_ok = mg_ok;
+    for (void *p : {(void *)c->xt, (void *)c->xt2}) {
+        if (p) {
+            (void)hipFree(p);
+            c->owned.erase(std::remove(c->owned.begin(), c->owned.end(), p), c->owned.end());
+        }
+    }
+    c->xt = c->xt2 = nullptr;
+    const size_t nv = std::max(c->levels.back().neq, 1);
+    if (dev_alloc(c, c->owned, &c->xt, nv) || dev_alloc(c, c->owned, &c->xt2, nv)) return MCL_E_DEVICE;
+    return 0;
+}
+
 extern "C" int mcl_mesh_upload(mcl_ctx *c, const mcl_mesh_view *levels, int nlev)
 {
     if (!c) return MCL_E_ARG;
@@ -854,6 +978,7 @@ extern "C" int mcl_mesh_upload(mcl_ctx *c, const mcl_mesh_view *levels, int nlev
     c->nlev = nlev;
     {
         int rc = build_multigrid(c);
+        if (!rc) rc = build_stationary(c);
         if (rc) return rc;
     }
     {
@@ -933,8 +1058,9 @@ extern "C" int mcl_set_control(mcl_ctx *c, const mcl_control *ctl)
 {
     if (!c || !ctl) return MCL_E_ARG;
     if (ctl->solver < 0 || ctl->solver > 3) return fail(c, MCL_E_ARG, "solver %d", ctl->solver);
-    if (ctl->solver == MCL_SOLVER_JC || ctl->solver == MCL_SOLVER_GS || ctl->solver == MCL_SOLVER_MG)
-        return fail(c, MCL_E_UNSUPPORTED, "inner solver %d not available on the device path yet", ctl->solver);
+    if (ctl->solver == MCL_SOLVER_MG && (ctl->mg_cycle < 0 || ctl->mg_finest_iteration < 0 ||
+                                         ctl->mg_coarse_iteration < 0))
+        return fail(c, MCL_E_ARG, "negative multigrid cycle or iteration count");
     if (ctl->precision != MCL_PREC_FP64 && ctl->precision != MCL_PREC_MIXED)
         return fail(c, MCL_E_ARG, "precision %d", ctl->precision);
     if (ctl->theta == 0.0) return fail(c, MCL_E_ARG, "theta = 0 divides by zero in the heat step");
@@ -1217,6 +1343,24 @@ static int zero_async(mcl_ctx *c, void *p, size_t bytes, bool by_kernel)
     return 0;
 }
 
+// Launch a persistent kernel: its workgroups wait on each other, so all of
+// them must be resident at once, and the grid never exceeds what the occupancy
+// query says fits (coop_blocks).  A cooperative launch guarantees residency;
+// the runtime refuses it (hipErrorIllegalState) while another process holds
+// the device's cooperative-launch resources (a second process on the same
+// GPU), and then the same grid is launched plainly: with the CUs free every
+// workgroup is dispatched at once, and a workgroup kept waiting longer than the
+// barriers' spin limit ends the solve with MCL_E_DEVICE instead of a hang.
+static hipError_t launch_persistent(mcl_ctx *c, const void *fn, unsigned nb, void **args)
+{
+    hipError_t e = hipLaunchCooperativeKernel(fn, dim3(nb), dim3(MCL_BLOCK), args, 0, c->stream);
+    if (e == hipErrorIllegalState) {
+        (void)hipGetLastError();
+        e = hipLaunchKernel(fn, dim3(nb), dim3(MCL_BLOCK), args, 0, c->stream);
+    }
+    return e;
+}
+
 static int reset_sync_if_aborted(mcl_ctx *c, const DevStats &st, const SolveWork &W, bool by_kernel)
 {
     if (st.aborted) {
@@ -1266,6 +1410,17 @@ static UzawaParams uzawa_params(mcl_ctx *c, double accuracy, const SolveWork &W)
     P.schur = k.schur;
     P.exact_dots = k.exact_dots;
     P.gs.nblocks = coop_blocks(c, std::max(L.neq, L.nel), MCL_FP64_ROWS_PER_THREAD * MCL_BLOCK, "MCL_FP64_BLOCKS");
+    P.solver = k.solver;
+    P.mg_cycle = k.mg_cycle;
+    P.mg_finest_iteration = k.mg_finest_iteration;
+    P.mg_coarse_iteration = k.mg_coarse_iteration;
+    P.xt = c->xt;
+    P.xt2 = c->xt2;
+    P.gsl = c->gs_tab;
+    P.ngsl = c->nlev;
+    // Gauss-Seidel sweeps and the multigrid V-cycle are sequential in the
+    // row order: the whole solve runs in one workgroup
+    if (k.solver == MCL_SOLVER_GS || k.solver == MCL_SOLVER_MG) P.gs.nblocks = 1;
     return P;
 }
 
@@ -1671,6 +1826,12 @@ static int solve_run(mcl_ctx *c, mcl_stokes_stats *out, const SolveWork &W, bool
     Level &L = c->levels.back();
     const mcl_control &k = c->ctl;
     int rc;
+    if (k.precision == MCL_PREC_FP64 && k.solver == MCL_SOLVER_MG) {
+        // Solver.c:678-683
+        if (c->nlev < 2) return fail(c, MCL_E_ARG, "for MG solver, min_level needs to be smaller than max_level-1");
+        if (!c->gs_mg_ok)
+            return fail(c, MCL_E_ARG, "solver mg needs the interp and project maps of every level above the coarsest");
+    }
     // initial_d_p (Solver.c:251-278): d = 0 on every level; p = 0 unless a
     // previous call exists, AMR is off and zero_P is off.
     if (!k.warm_start) {
@@ -1719,13 +1880,11 @@ static int solve_run(mcl_ctx *c, mcl_stokes_stats *out, const SolveWork &W, bool
         if (rc) return rc;
         void *margs[] = {&M};
         if (!noalloc) HIPCHK(c, hipEventRecord(ev.a, c->stream));
-        HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_uzawa_mixed, dim3(M.U.gs.nblocks), dim3(MCL_BLOCK), margs,
-                                             0, c->stream));
+        HIPCHK(c, launch_persistent(c, (const void *)k_uzawa_mixed, M.U.gs.nblocks, margs));
     } else {
         void *args[] = {&P};
         if (!noalloc) HIPCHK(c, hipEventRecord(ev.a, c->stream));
-        HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_uzawa_fp64, dim3(P.gs.nblocks), dim3(MCL_BLOCK), args,
-                                             0, c->stream));
+        HIPCHK(c, launch_persistent(c, (const void *)k_uzawa_fp64, P.gs.nblocks, args));
     }
     if (!noalloc) HIPCHK(c, hipEventRecord(ev.b, c->stream));
     // nodal velocity and stress (Solver.c:8-9)
@@ -1812,6 +1971,9 @@ extern "C" int mcl_solve_velocity_pressure_noalloc(mcl_ctx *c, mcl_stokes_stats 
     if (rc) return rc;
     if (c->ctl.precision != MCL_PREC_FP64)
         return fail(c, MCL_E_UNSUPPORTED, "the allocation-free solve runs the FP64 Uzawa kernel only (precision 0)");
+    if (c->ctl.solver != MCL_SOLVER_CG)
+        return fail(c, MCL_E_UNSUPPORTED, "the allocation-free solve supports the conjugate-gradient inner solver only "
+                                          "(solver cg)");
     HIPCHK(c, hipSetDevice(c->device));
     if (c->device < 0 || c->device >= MCL_SCRATCH_DEVICES || !g_scratch[c->device])
         return fail(c, MCL_E_DEVICE, "static scratch of device %d not located: %s", c->device,
@@ -1884,8 +2046,7 @@ extern "C" int mcl_update_temperature(mcl_ctx *c, mcl_heat_stats *out)
     HIPCHK(c, hipEventCreate(&e1));
     HIPCHK(c, hipEventRecord(e0, c->stream));
     void *args[] = {&H};
-    HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_heat_fp64, dim3(H.gs.nblocks), dim3(MCL_BLOCK), args, 0,
-                                         c->stream));
+    HIPCHK(c, launch_persistent(c, (const void *)k_heat_fp64, H.gs.nblocks, args));
     HIPCHK(c, hipEventRecord(e1, c->stream));
     DevStats st;
     double mt;
@@ -2116,8 +2277,7 @@ static int uzawa_probe(mcl_ctx *c, double *d, double *p, double *r1)
         M.inner_rtol = e && atof(e) > 0 ? atof(e) : 1e-3;
     }
     void *margs[] = {&M};
-    HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_uzawa_probe, dim3(M.U.gs.nblocks), dim3(MCL_BLOCK), margs, 0,
-                                         c->stream));
+    HIPCHK(c, launch_persistent(c, (const void *)k_uzawa_probe, M.U.gs.nblocks, margs));
     DevStats st;
     HIPCHK(c, hipMemcpyAsync(&st, c->stats, sizeof st, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

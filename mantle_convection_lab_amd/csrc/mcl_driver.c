@@ -103,6 +103,9 @@ int main(int argc, char **argv)
             }
         }
         if ((rc = mcl_update_temperature(ctx, &hs))) return die(ctx, "mcl_update_temperature", rc);
+        /* the first step's initial_T_dot solve prints its own line first
+         * (Update_temperature.c:1128, 1255) */
+        if (step == 0) printf("Temperature CG cycle = %d\n", hs.init_cg_cycles);
         printf("Temperature CG cycle = %d\n", hs.cg_cycles);
         fflush(stdout);
     }

@@ -402,6 +402,186 @@ __device__ int pcg_persistent(const LevelDev &L, const EllMat &A, const double *
     return cycle;
 }
 
+// ------------------------------------------------ stationary inner solvers
+// The reference's solver="jc", "gs" and "mg" inner solves (Solver.c:82-89),
+// with its arithmetic: every row sum in ascending-column order with the
+// diagonal left out (prod += a_ij x_j, x_i = (b_i - prod) / a_ii), the stop on
+// max |x - x_old| < accu, no hanging-node constraint.
+
+// jacobi (Solver.c:1800-1864) over the whole grid: sweep k reads one buffer
+// and writes the other (x_old needs no copy); the sweep maxima alternate
+// between two reduction slots, so a workgroup writing a slot two sweeps later
+// cannot overtake a slower one still reading it.
+__device__ int jacobi_grid(const EllMat &A, const double *__restrict__ b, double *x, double *xt, double accu,
+                           int max_cycle, const GridSync &gs, unsigned &gen, const RedBuf &rb, double *ldsm, bool &ok,
+                           long long &spmv)
+{
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsize = gridDim.x * blockDim.x;
+    const int n = A.n;
+    double *src = x, *dst = xt;
+    int cycle = 0;
+    while (ok && cycle < max_cycle) {
+        double mx = 0.0;
+        for (int i = gtid; i < n; i += gsize) {
+            double prod = 0.0, diag = 1.0;
+            const int nc = A.ncol[i];
+            for (int j = 0; j < nc; j++) {
+                const int c = A.col[j * n + i];
+                const double v = A.val[j * n + i];
+                if (c != i) prod += v * src[c];
+                else diag = v;
+            }
+            const double xn = (b[i] - prod) / diag;
+            dst[i] = xn;
+            mx = fmax(mx, fabs(xn - src[i]));
+        }
+        spmv++;
+        const int slot = 4 + (cycle & 1);
+        double bm = block_max(mx, ldsm);
+        if (threadIdx.x == 0) rb.pmax[slot * MCL_MAX_WG + blockIdx.x] = bm;
+        ok = ok && grid_sync(gs, gen);
+        double m = 0.0;
+        for (int bb = threadIdx.x; bb < (int)gridDim.x; bb += MCL_BLOCK) m = fmax(m, rb.pmax[slot * MCL_MAX_WG + bb]);
+        const double diff = block_max(m, ldsm);
+        double *t = src;
+        src = dst;
+        dst = t;
+        cycle++;
+        if (diff < accu) break;
+    }
+    if (src != x) {
+        for (int i = gtid; i < n; i += gsize) x[i] = src[i];
+        ok = ok && grid_sync(gs, gen);
+    }
+    return cycle;
+}
+
+// gauss_seidel (Solver.c:1866-1928) in one workgroup: each sweep walks the
+// level's wavefront groups in order, the rows of a group in parallel.
+__device__ int gs_block(const GsLevelDev &A, const double *b, double *x, double *xt, double accu, int max_cycle,
+                        double *ldsm, long long &spmv)
+{
+    const int t = threadIdx.x, n = A.n;
+    int cycle = 0;
+    while (cycle < max_cycle) {
+        for (int i = t; i < n; i += blockDim.x) xt[i] = x[i];
+        __syncthreads();
+        for (int g = 0; g < A.nsched; g++) {
+            const int k1 = A.sptr[g + 1];
+            for (int k = A.sptr[g] + t; k < k1; k += blockDim.x) {
+                const int i = A.srows[k];
+                double prod = 0.0, diag = 1.0;
+                const int nc = A.ncol[i];
+                for (int j = 0; j < nc; j++) {
+                    const int c = A.col[j * n + i];
+                    const double v = A.val[j * n + i];
+                    if (c != i) prod += v * x[c];
+                    else diag = v;
+                }
+                x[i] = (b[i] - prod) / diag;
+            }
+            __syncthreads();
+        }
+        spmv++;
+        cycle++;
+        double mx = 0.0;
+        for (int i = t; i < n; i += blockDim.x) mx = fmax(mx, fabs(x[i] - xt[i]));
+        if (block_max(mx, ldsm) < accu) break;
+    }
+    return cycle;
+}
+
+// multigrid + MG (Solver.c:668-769) in one workgroup, levels coarsest first
+// (lv[0] = min_level, lv[nl-1] = the finest).  Below the finest the iterate
+// is the level's err vector itself, so the prolongation overwrites the
+// smoothed iterate before it is added, as in the reference.  err keeps its
+// values across V-cycles and is zeroed once per call (initial_error).
+__device__ int mg_block(const GsLevelDev *lv, int nl, const double *b, double *x, double *xt_cycle, double *xt,
+                        double accu, int mg_cycle, int fi, int ci, double *ldsm, long long &spmv)
+{
+    const int t = threadIdx.x, top = nl - 1;
+    for (int l = 0; l < nl; l++)
+        for (int i = t; i < lv[l].n; i += blockDim.x) lv[l].err[i] = 0.0;
+    __syncthreads();
+    const int n = lv[top].n;
+    int cycle = 0;
+    bool conv = false;
+    while (!conv && cycle < mg_cycle) {
+        for (int i = t; i < n; i += blockDim.x) xt_cycle[i] = x[i];
+        __syncthreads();
+        for (int l = top; l >= 1; l--) {
+            const GsLevelDev &A = lv[l];
+            const double *rhs = l == top ? b : A.rhs;
+            double *xl = l == top ? x : A.err;
+            gs_block(A, rhs, xl, xt, accu, fi, ldsm, spmv);
+            // get_residual (Solver.c:1291-1316): res = b, then res -= a_ij x_j in column order
+            for (int i = t; i < A.n; i += blockDim.x) {
+                double r = rhs[i];
+                const int nc = A.ncol[i];
+                for (int j = 0; j < nc; j++) r -= A.val[j * A.n + i] * xl[A.col[j * A.n + i]];
+                A.resu[i] = r;
+            }
+            __syncthreads();
+            // project_res (Solver.c:770-832): injection
+            for (int k = t; k < A.nproj; k += blockDim.x) lv[l - 1].rhs[A.proj_l[k]] = A.resu[A.proj_u[k]];
+            __syncthreads();
+        }
+        gs_block(lv[0], lv[0].rhs, lv[0].err, xt, accu, 1000, ldsm, spmv);
+        for (int l = 1; l <= top; l++) {
+            const GsLevelDev &A = lv[l];
+            const double *rhs = l == top ? b : A.rhs;
+            double *xl = l == top ? x : A.err;
+            const double *ec = lv[l - 1].err;
+            // interp_error (Solver.c:1009-1143, local part)
+            for (int u = t; u < A.n; u += blockDim.x) {
+                const int m = A.pn[u];
+                if (m > 0) {
+                    double v = 0.0;
+                    for (int a = 0; a < m; a++) {
+                        const int c = A.pcol[a * A.n + u];
+                        if (c != -1) v += ec[c];
+                    }
+                    A.err[u] = v / m;
+                }
+            }
+            __syncthreads();
+            for (int i = t; i < A.n; i += blockDim.x) xl[i] += A.err[i];
+            __syncthreads();
+            gs_block(A, rhs, xl, xt, accu, ci, ldsm, spmv);
+        }
+        double mx = 0.0;
+        for (int i = t; i < n; i += blockDim.x) mx = fmax(mx, fabs(x[i] - xt_cycle[i]));
+        conv = block_max(mx, ldsm) < accu;
+        cycle++;
+    }
+    return cycle;
+}
+
+// The inner solve of citcom_solver (Solver.c:82-89) for the chosen solver.
+__device__ int inner_solve(const UzawaParams &P, const LevelDev &L, const EllMat &K, const double *b, double *x,
+                           bool x_zero, CgVecs cv, unsigned &gen, dd (*lds)[MCL_BLOCK / MCL_WAVE], double *ldsm,
+                           bool &ok, long long &spmv)
+{
+    switch (P.solver) {
+    case MCL_INNER_JC:
+        return jacobi_grid(K, b, x, P.xt, P.accuracy, 1000, P.gs, gen, P.rb, ldsm, ok, spmv);
+    case MCL_INNER_GS: {
+        const int c = gs_block(P.gsl[P.ngsl - 1], b, x, P.xt, P.accuracy, 1000, ldsm, spmv);
+        ok = ok && grid_sync(P.gs, gen);
+        return c;
+    }
+    case MCL_INNER_MG: {
+        const int c = mg_block(P.gsl, P.ngsl, b, x, P.xt2, P.xt, P.accuracy, P.mg_cycle, P.mg_finest_iteration,
+                               P.mg_coarse_iteration, ldsm, spmv);
+        ok = ok && grid_sync(P.gs, gen);
+        return c;
+    }
+    default:
+        return pcg_persistent(L, K, b, x, x_zero, P.accuracy, P.max_cg_cycle, P.cg_precondition, 0, cv, P.gs, gen,
+                              P.rb, lds, ldsm, ok, spmv, P.exact_dots != 0);
+    }
+}
+
 // citcom_solver (Solver.c:14-249) as one persistent kernel.
 __global__ void __launch_bounds__(MCL_BLOCK) k_uzawa_fp64(UzawaParams P)
 {
@@ -437,8 +617,7 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_uzawa_fp64(UzawaParams P)
     }
     spmv++;
     ok = ok && grid_sync(P.gs, gen);
-    inner += pcg_persistent(L, K, P.Ahv, P.u1, false, P.accuracy, P.max_cg_cycle, P.cg_precondition, 0, cv,
-                            P.gs, gen, P.rb, lds, ldsm, ok, spmv, P.exact_dots != 0);
+    inner += inner_solve(P, L, K, P.Ahv, P.u1, false, cv, gen, lds, ldsm, ok, spmv);
     for (int i = gtid; i < neq; i += gsize) P.d[i] += P.u1[i];
     ok = ok && grid_sync(P.gs, gen);
     double *r0 = P.r0, *r1 = P.r1, *r2 = P.r2, *z0 = P.z0, *z1 = P.z1, *s1 = P.s1, *s2 = P.s2;
@@ -479,8 +658,7 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_uzawa_fp64(UzawaParams P)
             P.u1[i] = 0.0;
         }
         ok = ok && grid_sync(P.gs, gen);
-        inner += pcg_persistent(L, K, P.Ahv, P.u1, true, P.accuracy, P.max_cg_cycle, P.cg_precondition, 0, cv,
-                                P.gs, gen, P.rb, lds, ldsm, ok, spmv, P.exact_dots != 0);
+        inner += inner_solve(P, L, K, P.Ahv, P.u1, true, cv, gen, lds, ldsm, ok, spmv);
         // Ah = G^T u1 ; alpha = r1.z1 / s2.Ah
         {
             dd acc[1] = {dd_zero()};

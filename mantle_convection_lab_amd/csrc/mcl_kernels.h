@@ -72,6 +72,33 @@ struct DevStats {
     long long t_begin, t_end;      // device wall clock (wall_clock64) at the FP64 Uzawa kernel's start and end
 };
 
+// One quadtree level for the reference's stationary inner solvers
+// (Gauss-Seidel and the stand-alone multigrid, Solver.c:668-1316,1866-1928):
+// the level's FP64 K in ELL form, the Gauss-Seidel wavefront schedule (rows
+// grouped so that every row comes after each lower-numbered row it couples to
+// and before each higher-numbered one: updating a group's rows in parallel
+// reproduces the sequential sweep exactly), the transfer maps and the level's
+// vectors.
+struct GsLevelDev {
+    int n, WK;
+    const int *col, *ncol;   // [WK][n], [n]
+    const double *val;       // [WK][n]
+    int nsched;
+    const int *sptr;         // [nsched + 1]
+    const int *srows;        // [n] rows of group g: srows[sptr[g] .. sptr[g+1])
+    // interp from the level below: row u <- mean of pn[u] entries pcol[a][u] (-1 skipped, still counted)
+    const int *pn, *pcol;    // [n], [4][n]
+    // injection into the level below: its row proj_l[k] <- this level's row proj_u[k]
+    int nproj;
+    const int *proj_l, *proj_u;
+    double *err, *rhs, *resu;   // [n]: correction (G.err[lev]), right-hand side, residual
+};
+
+#define MCL_INNER_JC 0
+#define MCL_INNER_GS 1
+#define MCL_INNER_CG 2
+#define MCL_INNER_MG 3
+
 struct UzawaParams {
     LevelDev L;
     GridSync gs;
@@ -83,6 +110,12 @@ struct UzawaParams {
     double accuracy, citcom_accu;
     int max_citcom_cycle, max_cg_cycle, cg_precondition, check_v_only, schur;
     int exact_dots;          // dot products in the reference's sequential order
+    // inner solver (MCL_INNER_*): Jacobi runs on the whole grid; Gauss-Seidel
+    // and multigrid need a launch of one workgroup
+    int solver, mg_cycle, mg_finest_iteration, mg_coarse_iteration;
+    double *xt, *xt2;        // [n_v] previous iterates (Jacobi ping-pong, sweep and V-cycle differences)
+    const GsLevelDev *gsl;   // [ngsl] levels, coarsest first (device memory)
+    int ngsl;
 };
 
 struct HeatParams {

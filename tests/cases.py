@@ -6,6 +6,9 @@ from tests.oracle_lib import fixture_keys, load_fixture
 # fixtures with a full first step (matrices, vectors, counts)
 STEP_CASES = ["c1_64", "c2s_32", "plain_cg_16", "dev12_32", "dev8_16", "dev14_16", "c2s_64"]
 MESH_CASES = STEP_CASES + ["mg_16"]
+# cases with the reference's other inner solvers (Jacobi, Gauss-Seidel, the
+# stand-alone multigrid) and auto_accuracy
+SOLVER_CASES = ["jc_16", "gs_16", "auto_16", "mg_16", "mg_32"]
 
 
 def case_input(name):

@@ -5,13 +5,15 @@ Every hot-path call goes through the C-ABI of libmcl_hip.so.  Bars:
     K/C/F, buoyancy; viscosity to 1 ulp where exp() is evaluated on the device);
   * one Stokes solve: identical Uzawa cycle count, u and p within 1e-10
     relative L2 of the reference (the only non-reference arithmetic is the
-    compensated parallel dot products);
+    compensated parallel dot products), with each of the reference's inner
+    solvers: Jacobi-PCG and plain CG, Jacobi, Gauss-Seidel and the stand-alone
+    multigrid V-cycle;
   * one heat step: T within 1e-10 relative L2, dt and model time to 1e-12.
 """
 import numpy as np
 import pytest
 
-from tests.cases import STEP_CASES, case_input, rel_l2
+from tests.cases import SOLVER_CASES, STEP_CASES, case_input, rel_l2
 from tests.oracle_lib import TW, VW, csr_from_slots
 
 pytestmark = pytest.mark.gpu
@@ -28,7 +30,7 @@ def _model(name):
     return fx, Model(inp, T0=T0)
 
 
-@pytest.mark.parametrize("name", STEP_CASES)
+@pytest.mark.parametrize("name", STEP_CASES + SOLVER_CASES)
 def test_first_step_matches_reference(name):
     fx, m = _model(name)
     c = m.ctx
@@ -92,7 +94,7 @@ def test_first_step_matches_reference(name):
     assert rel_l2(c.get_field(lev, "HEAT_D"), fx["s0.heat.d"]) < TOL
 
 
-@pytest.mark.parametrize("name", ["c1_64", "c2s_32", "plain_cg_16"])
+@pytest.mark.parametrize("name", ["c1_64", "c2s_32", "plain_cg_16", "jc_16", "gs_16", "mg_32"])
 def test_second_step_matches_reference(name):
     fx, m = _model(name)
     r0 = m.step()
