@@ -104,22 +104,27 @@ def fp64_bytes(rec, sizes, w=8, iw=4):
 
 
 def mixed_bytes(rec, sizes, pre=MG_SWEEPS):
-    """Algorithmic bytes of k_uzawa_mixed for one solve.  Per inner iteration:
+    """Algorithmic bytes of k_uzawa_mixed for one solve: what the algorithm
+    must move at the least, not what the kernel moves.  Per inner iteration:
     FP64 part  nnz_K 12 (K d' with 8-byte values, 4-byte columns)
                + n_v (4 + 8 + 8 + 8)        (read z32, d; write d', q)
                + n_v (4*8 + 2*8 + 3*4)      (x, r, d', q read; x, r written;
                                              b32, x32 written, 1/diag read);
-    V-cycle    per level l with n_l rows and nnz_l entries (FP32 values,
-               int32 columns): 2 pre passes over K32 (sweeps 2..pre, the
-               residual, the post-sweeps) at nnz_l 8 + 3 n_l 4 each, plus the
-               first sweep 3 n_l 4, restriction and prolongation 6 n_l 4
-               (levels below the finest counted the same way);
-    plus the Uzawa iterations as in fp64_bytes."""
+    V-cycle    per level l above the coarsest (n_l rows, nnz_l entries, FP32
+               values and int32 columns): the level's K once per leg, 2 nnz_l 8
+               (a leg's `pre` sweeps and its residual reuse the rows it read),
+               the down leg reading b and 1/diag and writing x, 12 n_l, the up
+               leg reading x, b, 1/diag and writing x, 16 n_l, and the transfers
+               writing the coarse right-hand side and reading the coarse
+               correction, 8 n_(l-1);
+    plus the Uzawa iterations as in fp64_bytes.  (`pre` does not enter: more
+    sweeps per leg reuse the same rows.)"""
     n_v, nnz, n_p = sizes[-1]
     b_fp64 = nnz * 12 + n_v * 28 + n_v * 60
     b_v = 0
-    for (n, z, _) in sizes[1:] if len(sizes) > 1 else sizes:
-        b_v += 2 * pre * (z * 8 + 3 * n * 4) + 3 * n * 4 + 6 * n * 4
+    for k in range(1, len(sizes)):
+        n, z, _ = sizes[k]
+        b_v += 2 * z * 8 + 28 * n + 8 * sizes[k - 1][0]
     b_uz = 2 * (4 * n_v) * 12 + 7 * n_v * 8 + 12 * n_p * 8
     return rec["inner"] * (b_fp64 + b_v) + rec["uzawa"] * b_uz
 
@@ -357,6 +362,10 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy leg (and the fp64 denominator)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--workload", default="c2", choices=["c2", "batched"],
+                    help="c2: the 256x256 metric above (default); batched: the 96-instance workload, one line of "
+                         "aggregate DOF-updates/s and per-instance step-time percentiles")
+    ap.add_argument("--lanes", type=int, default=4, help="batched: concurrent streams of the graph rounds")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -369,6 +378,9 @@ def main():
     def log(s):
         print(s, file=sys.stderr, flush=True)
 
+    if args.workload == "batched":
+        batched_line(args, log)
+        return
     log(f"[bench] {SIDE}x{SIDE}, primary path {args.path}")
     net, ppath, note = load_net(args.path, log)
     prim = run_timed(ppath, net, args.steps, args.warmup, log, note)
@@ -426,6 +438,33 @@ def main():
         first = (fp["inner_iterations"][0], fp["uzawa_cycles"][0]) if fp and fp["inner_iterations"] else (342793, 310)
         out["cpu_baseline"] = cpu_baseline(*first)
     print(json.dumps(out), flush=True)
+
+
+def batched_line(args, log):
+    """The batched workload: 96 seeded instances (sides 32-512, Ra 1e4-1e7,
+    contrast 1-1e3), `steps` rounds (default 5): round 0 instance by instance
+    with the learned guess on sides >= 128, then graph rounds on `lanes`
+    streams.  value = aggregate DOF-updates/s (velocity + pressure +
+    temperature unknowns of every instance per round, over the rounds' wall
+    time); the sequential form of the same rounds (lanes 0) is measured
+    after it for comparison unless --no-accuracy is given."""
+    from mantle_convection_lab_amd import batch
+    steps = args.steps if args.steps != 100 else 5
+    log(f"[bench] batched workload: 96 instances, {steps} rounds, lanes {args.lanes}")
+    out = batch.run(96, steps, lanes=args.lanes, log=log)
+    seq = None
+    if not args.no_accuracy:
+        log("[bench] the same rounds instance by instance (lanes 0)")
+        seq = batch.run(96, steps, lanes=0, log=log)
+    line = {"metric": "aggregate DOF-updates/s, 96 batched instances", "value": out["dof_updates_per_s"],
+            "unit": "DOF-updates/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+            "ms_per_step": out["wall_s"] / max(steps, 1) * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic (seeded instances)",
+            "config": {"workload": "96 seeded instances, sides 32-512, Ra 1e4-1e7, contrast 1-1e3, mixed "
+                                   "precision, learned guess on sides >= 128 at the cold start",
+                       "batched": out, "sequential": seq,
+                       "speedup_vs_sequential": (out["dof_updates_per_s"] / seq["dof_updates_per_s"]) if seq else None}}
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

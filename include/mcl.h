@@ -165,7 +165,9 @@ typedef struct mcl_stokes_stats {
     long long inner_iterations;
     long long spmv;
     double accuracy_used;
-    double res_momentum;       /* |F - K d - G p| / |F| after the solve */
+    double res_momentum;       /* |F - K d - G p| / |F| after the solve (the mixed path on meshes
+                                  with hanging nodes: |C^T (F - K d - G p)| / |C^T F| in the
+                                  constrained space it solves in) */
     double res_continuity;     /* |G^T d| / |F| after the solve */
     double device_ms;          /* device time of the persistent Uzawa kernel (HIP events on its stream) */
 } mcl_stokes_stats;
@@ -207,6 +209,34 @@ int mcl_update_temperature(mcl_ctx *ctx, mcl_heat_stats *stats);        /* Updat
 
 /* Synchronise the context's stream. */
 int mcl_synchronize(mcl_ctx *ctx);
+
+/* Most workgroups any persistent kernel of this context may launch (0: as
+ * many as fit on the device, the default).  For contexts that share a device
+ * with other work at the same time; results depend on the workgroup count
+ * only through the order of the parallel sums. */
+int mcl_set_launch_cap(mcl_ctx *ctx, int max_workgroups);
+
+/* ---------------------------------------------------------------------------
+ * Batches (no counterpart in the reference, which advances one instance per
+ * job): many independent instances, one context each on the same device,
+ * advanced together.  mcl_batch_create captures each instance's time step
+ * (buoyancy, viscosity, assembly, the Stokes solve with nodal velocity and
+ * stress, the heat step) into a HIP graph; mcl_batch_step replays every
+ * instance's graph on one of `lanes` concurrent streams and waits for all.
+ * Every instance must have taken its first step through the ordinary entry
+ * points; auto_accuracy is not available.  While the batch exists, each of its
+ * contexts launches at most (device capacity / lanes) workgroups per
+ * persistent kernel (or its own smaller launch cap), so the persistent kernels
+ * of all lanes fit on the device at once. */
+typedef struct mcl_batch mcl_batch;
+int mcl_batch_create(mcl_ctx *const *ctxs, int n, int lanes, mcl_batch **out);
+void mcl_batch_destroy(mcl_batch *batch);
+const char *mcl_batch_last_error(const mcl_batch *batch);
+/* one time step of every instance with skip[i] == 0 (skip may be NULL); per
+ * instance the Stokes and heat statistics and the step's device time (ms,
+ * events around its graph on its lane); any output array may be NULL */
+int mcl_batch_step(mcl_batch *batch, const int *skip, mcl_stokes_stats *stokes, mcl_heat_stats *heat,
+                   double *step_ms);
 
 /* mcl_solve_velocity_pressure without any allocation.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP

@@ -129,6 +129,13 @@ def hip_lib():
         L.mcl_solve_velocity_pressure.argtypes = [vp, C.POINTER(StokesStats)]
         L.mcl_solve_velocity_pressure_noalloc.argtypes = [vp, C.POINTER(StokesStats)]
         L.mcl_update_temperature.argtypes = [vp, C.POINTER(HeatStats)]
+        L.mcl_set_launch_cap.argtypes = [vp, C.c_int]
+        L.mcl_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.POINTER(vp)]
+        L.mcl_batch_destroy.argtypes = [vp]
+        L.mcl_batch_destroy.restype = None
+        L.mcl_batch_last_error.argtypes = [vp]
+        L.mcl_batch_last_error.restype = C.c_char_p
+        L.mcl_batch_step.argtypes = [vp, PI, C.POINTER(StokesStats), C.POINTER(HeatStats), PD]
         _hip = L
     return _hip
 

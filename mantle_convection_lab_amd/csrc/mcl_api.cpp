@@ -82,6 +82,7 @@ struct mcl_ctx {
     bool first_heat = true;
     bool been_here_dp = false;
     int max_coop_blocks = 0;
+    int block_cap = 0;               // > 0: most workgroups of any persistent launch (set by a batch)
     int num_cu = 0;
     int wall_khz = 0;                // rate of the device wall clock (wall_clock64)
     // solver work space (finest level)
@@ -1210,6 +1211,13 @@ extern "C" int mcl_synchronize(mcl_ctx *c)
     return 0;
 }
 
+extern "C" int mcl_set_launch_cap(mcl_ctx *c, int max_workgroups)
+{
+    if (!c || max_workgroups < 0) return MCL_E_ARG;
+    c->block_cap = max_workgroups;
+    return 0;
+}
+
 static int ready(mcl_ctx *c)
 {
     if (!c) return MCL_E_ARG;
@@ -1277,7 +1285,7 @@ extern "C" int mcl_buoyancy_force(mcl_ctx *c)
     return 0;
 }
 
-static int mixed_prepare(mcl_ctx *c);
+static int mixed_prepare(mcl_ctx *c, bool by_kernel);
 
 extern "C" int mcl_velocity_pressure_KGF_matrix(mcl_ctx *c)
 {
@@ -1290,7 +1298,7 @@ extern "C" int mcl_velocity_pressure_KGF_matrix(mcl_ctx *c)
         HIPCHK(c, hipGetLastError());
     }
     c->mixed_ready = false;
-    if (c->ctl.precision == MCL_PREC_MIXED) return mixed_prepare(c);
+    if (c->ctl.precision == MCL_PREC_MIXED) return mixed_prepare(c, false);
     return 0;
 }
 
@@ -1309,7 +1317,9 @@ static unsigned coop_blocks(mcl_ctx *c, long long n, int rows_per_block, const c
     long long b = (n + per - 1) / per;
     const char *e = getenv(knob);
     if (e && atoi(e) > 0) b = atoi(e);
-    return (unsigned)std::max<long long>(1, std::min<long long>(b, c->max_coop_blocks));
+    long long cap = c->max_coop_blocks;
+    if (c->block_cap > 0) cap = std::min<long long>(cap, c->block_cap);
+    return (unsigned)std::max<long long>(1, std::min<long long>(b, cap));
 }
 
 // Device buffers a Stokes solve works in besides the mesh and the fields:
@@ -1351,8 +1361,11 @@ static int zero_async(mcl_ctx *c, void *p, size_t bytes, bool by_kernel)
 // GPU), and then the same grid is launched plainly: with the CUs free every
 // workgroup is dispatched at once, and a workgroup kept waiting longer than the
 // barriers' spin limit ends the solve with MCL_E_DEVICE instead of a hang.
-static hipError_t launch_persistent(mcl_ctx *c, const void *fn, unsigned nb, void **args)
+static hipError_t launch_persistent(mcl_ctx *c, const void *fn, unsigned nb, void **args, bool plain = false)
 {
+    // a batch's captured graphs launch plainly: their grids are capped so that
+    // the persistent kernels of all its lanes fit on the device together
+    if (plain) return hipLaunchKernel(fn, dim3(nb), dim3(MCL_BLOCK), args, 0, c->stream);
     hipError_t e = hipLaunchCooperativeKernel(fn, dim3(nb), dim3(MCL_BLOCK), args, 0, c->stream);
     if (e == hipErrorIllegalState) {
         (void)hipGetLastError();
@@ -1426,9 +1439,10 @@ static UzawaParams uzawa_params(mcl_ctx *c, double accuracy, const SolveWork &W)
 
 // FP32 copies of K and smoother bounds per multigrid level and the dense
 // inverse of the coarsest K: once per assembly (mixed precision).
-static int mixed_prepare(mcl_ctx *c)
+static int mixed_prepare(mcl_ctx *c, bool by_kernel)
 {
-    HIPCHK(c, hipMemsetAsync(c->lam_bits, 0, MCL_MAX_LEVELS * sizeof(unsigned), c->stream));
+    int rc = zero_async(c, c->lam_bits, MCL_MAX_LEVELS * sizeof(unsigned), by_kernel);
+    if (rc) return rc;
     if (c->prep_blocks > 0) {
         hipLaunchKernelGGL(k_prepare_fp32, dim3(c->prep_blocks), dim3(MCL_BLOCK), 0, c->stream,
                            (const PrepLevel *)c->prep_tab, c->prep_n);
@@ -1818,75 +1832,59 @@ struct EventPair {
     }
 };
 
-// solve_velocity_pressure (Solver.c:3-12) on work space W.  noalloc: the
-// allocation-free form (FP64 only; no events, device time from the kernel's
-// clock stamps; every buffer cleared by the zero-fill kernel).
-static int solve_run(mcl_ctx *c, mcl_stokes_stats *out, const SolveWork &W, bool noalloc)
+// How a solve or heat step is enqueued: NORMAL (the allocating entry points:
+// hipMemsetAsync, cooperative launch, HIP events), NOALLOC (nothing
+// allocated: buffers cleared by the zero-fill kernel, no events) or GRAPH
+// (captured into a batch's graph: zero-fill kernel, plain launch, no events,
+// no host synchronisation).
+enum EnqMode { ENQ_NORMAL, ENQ_NOALLOC, ENQ_GRAPH };
+
+// The device work of solve_velocity_pressure (Solver.c:3-12) on work space W,
+// up to the residual norms in W.resid; `accuracy` is the inner tolerance.
+static int solve_enqueue(mcl_ctx *c, const SolveWork &W, EnqMode mode, EventPair *ev, double accuracy)
 {
     Level &L = c->levels.back();
     const mcl_control &k = c->ctl;
+    const bool by_kernel = mode != ENQ_NORMAL, plain = mode == ENQ_GRAPH;
     int rc;
-    if (k.precision == MCL_PREC_FP64 && k.solver == MCL_SOLVER_MG) {
-        // Solver.c:678-683
-        if (c->nlev < 2) return fail(c, MCL_E_ARG, "for MG solver, min_level needs to be smaller than max_level-1");
-        if (!c->gs_mg_ok)
-            return fail(c, MCL_E_ARG, "solver mg needs the interp and project maps of every level above the coarsest");
-    }
     // initial_d_p (Solver.c:251-278): d = 0 on every level; p = 0 unless a
     // previous call exists, AMR is off and zero_P is off.
     if (!k.warm_start) {
         for (auto &Lv : c->levels) {
-            rc = zero_async(c, Lv.d, std::max(Lv.neq, 1) * sizeof(double), noalloc);
+            rc = zero_async(c, Lv.d, std::max(Lv.neq, 1) * sizeof(double), by_kernel);
             if (rc) return rc;
             if (!(c->been_here_dp && k.AMR == 0 && k.zero_P == 0)) {
-                rc = zero_async(c, Lv.p, Lv.nel * sizeof(double), noalloc);
+                rc = zero_async(c, Lv.p, Lv.nel * sizeof(double), by_kernel);
                 if (rc) return rc;
             }
         }
     }
     c->been_here_dp = true;
-    if (noalloc) {
-        // barrier words (the kernel starts from generation 0) and statistics
+    if (mode == ENQ_NOALLOC) {
+        // barrier words (the kernel starts from generation 0)
         rc = zero_async(c, W.gs.gen, MCL_SYNC_WORDS * sizeof(unsigned), true);
         if (rc) return rc;
     }
-    rc = zero_async(c, W.stats, sizeof(DevStats), noalloc);
+    rc = zero_async(c, W.stats, sizeof(DevStats), by_kernel);
     if (rc) return rc;
-    double accuracy = k.accuracy;
-    if (k.auto_accuracy) {
-        // Solver.c:41-51: accuracy = citcom_accu * sqrt(F.F / neq)
-        hipLaunchKernelGGL(k_residuals, dim3(grid_for(c, L.neq)), dim3(MCL_BLOCK), 0, c->stream, L.dev, W.rb,
-                           (const double *)L.d, (const double *)L.p);
-        hipLaunchKernelGGL(k_collect3, dim3(1), dim3(MCL_BLOCK), 0, c->stream, W.rb, 12,
-                           (int)grid_for(c, L.neq), W.resid);
-        double r[3];
-        HIPCHK(c, hipMemcpyAsync(r, W.resid, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        accuracy = k.citcom_accu * std::sqrt(r[2] / L.neq);
-    }
     UzawaParams P = uzawa_params(c, accuracy, W);
-    EventPair ev;
-    if (!noalloc) {
-        rc = ev.create(c);
-        if (rc) return rc;
-    }
     if (k.precision == MCL_PREC_MIXED) {
-        if (!c->mixed_ready) {
-            rc = mixed_prepare(c);
+        if (!c->mixed_ready || mode == ENQ_GRAPH) {
+            rc = mixed_prepare(c, by_kernel);
             if (rc) return rc;
         }
         MixedParams M;
         rc = mixed_params(c, P, M);
         if (rc) return rc;
         void *margs[] = {&M};
-        if (!noalloc) HIPCHK(c, hipEventRecord(ev.a, c->stream));
-        HIPCHK(c, launch_persistent(c, (const void *)k_uzawa_mixed, M.U.gs.nblocks, margs));
+        if (ev) HIPCHK(c, hipEventRecord(ev->a, c->stream));
+        HIPCHK(c, launch_persistent(c, (const void *)k_uzawa_mixed, M.U.gs.nblocks, margs, plain));
     } else {
         void *args[] = {&P};
-        if (!noalloc) HIPCHK(c, hipEventRecord(ev.a, c->stream));
-        HIPCHK(c, launch_persistent(c, (const void *)k_uzawa_fp64, P.gs.nblocks, args));
+        if (ev) HIPCHK(c, hipEventRecord(ev->a, c->stream));
+        HIPCHK(c, launch_persistent(c, (const void *)k_uzawa_fp64, P.gs.nblocks, args, plain));
     }
-    if (!noalloc) HIPCHK(c, hipEventRecord(ev.b, c->stream));
+    if (ev) HIPCHK(c, hipEventRecord(ev->b, c->stream));
     // nodal velocity and stress (Solver.c:8-9)
     hipLaunchKernelGGL(k_v_to_nodes, dim3(grid_for(c, L.nno)), dim3(MCL_BLOCK), 0, c->stream, L.dev,
                        (const double *)L.d, L.V);
@@ -1896,10 +1894,72 @@ static int solve_run(mcl_ctx *c, mcl_stokes_stats *out, const SolveWork &W, bool
                        (const double *)L.es, (const double *)L.area, L.stress);
     HIPCHK(c, hipGetLastError());
     hipLaunchKernelGGL(k_residuals, dim3(grid_for(c, L.neq)), dim3(MCL_BLOCK), 0, c->stream, L.dev, W.rb,
-                       (const double *)L.d, (const double *)L.p);
+                       (const double *)L.d, (const double *)L.p, k.precision == MCL_PREC_MIXED ? 1 : 0);
     hipLaunchKernelGGL(k_collect3, dim3(1), dim3(MCL_BLOCK), 0, c->stream, W.rb, 12, (int)grid_for(c, L.neq),
                        W.resid);
     HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+// Statistics of a finished solve from the kernel's block and the residual norms.
+static void solve_stats(mcl_ctx *c, const DevStats &st, const double *r, double accuracy, float ms,
+                        mcl_stokes_stats *out)
+{
+    const mcl_control &k = c->ctl;
+    if (!out) return;
+    out->uzawa_cycles = st.cycles;
+    out->dv = st.dv;
+    out->dp = st.dp;
+    out->converged = (st.dv <= k.citcom_accu && (k.check_v_only || st.dp <= k.citcom_accu)) ? 1 : 0;
+    out->inner_iterations = st.inner;
+    out->spmv = st.spmv;
+    out->accuracy_used = accuracy;
+    double fn = std::sqrt(r[2]);
+    out->res_momentum = fn > 0 ? std::sqrt(r[0]) / fn : std::sqrt(r[0]);
+    out->res_continuity = fn > 0 ? std::sqrt(r[1]) / fn : std::sqrt(r[1]);
+    out->device_ms = ms;
+}
+
+static int solve_checks(mcl_ctx *c)
+{
+    const mcl_control &k = c->ctl;
+    if (k.precision == MCL_PREC_FP64 && k.solver == MCL_SOLVER_MG) {
+        // Solver.c:678-683
+        if (c->nlev < 2) return fail(c, MCL_E_ARG, "for MG solver, min_level needs to be smaller than max_level-1");
+        if (!c->gs_mg_ok)
+            return fail(c, MCL_E_ARG, "solver mg needs the interp and project maps of every level above the coarsest");
+    }
+    return 0;
+}
+
+// solve_velocity_pressure (Solver.c:3-12) on work space W, synchronously.
+// noalloc: the allocation-free form (FP64 only; no events, device time from
+// the kernel's clock stamps; every buffer cleared by the zero-fill kernel).
+static int solve_run(mcl_ctx *c, mcl_stokes_stats *out, const SolveWork &W, bool noalloc)
+{
+    Level &L = c->levels.back();
+    const mcl_control &k = c->ctl;
+    int rc = solve_checks(c);
+    if (rc) return rc;
+    double accuracy = k.accuracy;
+    if (k.auto_accuracy) {
+        // Solver.c:41-51: accuracy = citcom_accu * sqrt(F.F / neq)
+        hipLaunchKernelGGL(k_residuals, dim3(grid_for(c, L.neq)), dim3(MCL_BLOCK), 0, c->stream, L.dev, W.rb,
+                           (const double *)L.d, (const double *)L.p, 0);
+        hipLaunchKernelGGL(k_collect3, dim3(1), dim3(MCL_BLOCK), 0, c->stream, W.rb, 12,
+                           (int)grid_for(c, L.neq), W.resid);
+        double r[3];
+        HIPCHK(c, hipMemcpyAsync(r, W.resid, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        accuracy = k.citcom_accu * std::sqrt(r[2] / L.neq);
+    }
+    EventPair ev;
+    if (!noalloc) {
+        rc = ev.create(c);
+        if (rc) return rc;
+    }
+    rc = solve_enqueue(c, W, noalloc ? ENQ_NOALLOC : ENQ_NORMAL, noalloc ? nullptr : &ev, accuracy);
+    if (rc) return rc;
     DevStats st;
     double r[3];
     HIPCHK(c, hipMemcpyAsync(&st, W.stats, sizeof st, hipMemcpyDeviceToHost, c->stream));
@@ -1915,23 +1975,11 @@ static int solve_run(mcl_ctx *c, mcl_stokes_stats *out, const SolveWork &W, bool
         int khz = 0;
         hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device);
         fprintf(stderr, "[mcl phases ms]");
-        for (int k = 0; k < MCL_PHASES; k++) fprintf(stderr, " %.2f", khz > 0 ? st.phase[k] / (double)khz : 0.0);
+        for (int q = 0; q < MCL_PHASES; q++) fprintf(stderr, " %.2f", khz > 0 ? st.phase[q] / (double)khz : 0.0);
         fprintf(stderr, "\n");
     }
 #endif
-    if (out) {
-        out->uzawa_cycles = st.cycles;
-        out->dv = st.dv;
-        out->dp = st.dp;
-        out->converged = (st.dv <= k.citcom_accu && (k.check_v_only || st.dp <= k.citcom_accu)) ? 1 : 0;
-        out->inner_iterations = st.inner;
-        out->spmv = st.spmv;
-        out->accuracy_used = accuracy;
-        double fn = std::sqrt(r[2]);
-        out->res_momentum = fn > 0 ? std::sqrt(r[0]) / fn : std::sqrt(r[0]);
-        out->res_continuity = fn > 0 ? std::sqrt(r[1]) / fn : std::sqrt(r[1]);
-        out->device_ms = ms;
-    }
+    solve_stats(c, st, r, accuracy, ms, out);
     if (st.nan_flag) return fail(c, MCL_E_NUMERIC, "NaN in the Stokes solve");
     return 0;
 }
@@ -2000,14 +2048,13 @@ extern "C" int mcl_solve_velocity_pressure_noalloc(mcl_ctx *c, mcl_stokes_stats 
     return solve_run(c, out, W, true);
 }
 
-extern "C" int mcl_update_temperature(mcl_ctx *c, mcl_heat_stats *out)
+// The device work of update_temperature (Update_temperature.c:3-133).
+static int heat_enqueue(mcl_ctx *c, EnqMode mode, EventPair *ev)
 {
-    int rc = ready(c);
-    if (rc) return rc;
-    HIPCHK(c, hipSetDevice(c->device));
     Level &L = c->levels.back();
     const mcl_control &k = c->ctl;
-    HIPCHK(c, hipMemsetAsync(c->stats, 0, sizeof(DevStats), c->stream));
+    int rc = zero_async(c, c->stats, sizeof(DevStats), mode != ENQ_NORMAL);
+    if (rc) return rc;
     HeatParams H{};
     H.L = L.dev;
     H.gs = c->gs;
@@ -2041,22 +2088,30 @@ extern "C" int mcl_update_temperature(mcl_ctx *c, mcl_heat_stats *out)
     H.cg_precondition = k.cg_precondition;
     H.exact_dots = k.exact_dots;
     H.gs.nblocks = coop_blocks(c, std::max(L.neqT, L.nel), MCL_FP64_ROWS_PER_THREAD * MCL_BLOCK, "MCL_FP64_BLOCKS");
-    hipEvent_t e0, e1;
-    HIPCHK(c, hipEventCreate(&e0));
-    HIPCHK(c, hipEventCreate(&e1));
-    HIPCHK(c, hipEventRecord(e0, c->stream));
+    if (ev) HIPCHK(c, hipEventRecord(ev->a, c->stream));
     void *args[] = {&H};
-    HIPCHK(c, launch_persistent(c, (const void *)k_heat_fp64, H.gs.nblocks, args));
-    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, launch_persistent(c, (const void *)k_heat_fp64, H.gs.nblocks, args, mode == ENQ_GRAPH));
+    if (ev) HIPCHK(c, hipEventRecord(ev->b, c->stream));
+    return 0;
+}
+
+extern "C" int mcl_update_temperature(mcl_ctx *c, mcl_heat_stats *out)
+{
+    int rc = ready(c);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    EventPair ev;
+    rc = ev.create(c);
+    if (rc) return rc;
+    rc = heat_enqueue(c, ENQ_NORMAL, &ev);
+    if (rc) return rc;
     DevStats st;
     double mt;
     HIPCHK(c, hipMemcpyAsync(&st, c->stats, sizeof st, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(&mt, c->model_time, sizeof mt, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    hipEventElapsedTime(&ms, ev.a, ev.b);
     rc = reset_sync_if_aborted(c, st, ctx_work(c), false);
     if (rc) return rc;
     c->first_heat = false;
@@ -2068,6 +2123,246 @@ extern "C" int mcl_update_temperature(mcl_ctx *c, mcl_heat_stats *out)
         out->device_ms = ms;
     }
     return 0;
+}
+
+// ------------------------------------------------------------ batches
+// Many independent instances advanced together on one device.  Each
+// instance's time step after its first -- buoyancy, viscosity, assembly (and
+// the FP32 multigrid hierarchy), the Stokes solve, nodal velocity and stress,
+// residual norms, the heat step, and the copies of its statistics into pinned
+// host memory -- is captured once into a HIP graph.  A round replays every
+// instance's graph on one of `lanes` streams (instances dealt largest first
+// to the least loaded lane, so the lanes carry similar work) and waits for
+// all; the lanes run concurrently and fill the device that one small
+// instance leaves mostly idle.  Every persistent launch of the batch's
+// instances is capped at max_coop_blocks / lanes workgroups, so the
+// persistent kernels of all lanes fit on the device together.
+namespace {
+struct HostSlot {
+    DevStats solve;
+    double resid[3];
+    DevStats heat;
+    double model_time;
+};
+}  // namespace
+
+struct mcl_batch {
+    int device = 0;
+    std::vector<mcl_ctx *> ctx;
+    std::vector<int> lane;               // lane of each instance
+    std::vector<hipStream_t> streams;    // one per lane
+    std::vector<hipGraphExec_t> exec;
+    std::vector<hipGraph_t> graph;
+    std::vector<hipEvent_t> ev0, ev1;    // around each instance's graph, on its lane
+    std::vector<double> accuracy;
+    std::vector<int> prev_cap;           // each instance's launch cap before the batch
+    HostSlot *slots = nullptr;           // pinned, one per instance
+    std::string err;
+};
+
+static int batch_fail(mcl_batch *b, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (b) b->err = buf;
+    return code;
+}
+
+extern "C" void mcl_batch_destroy(mcl_batch *b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    for (hipStream_t s : b->streams) (void)hipStreamSynchronize(s);
+    for (auto e : b->exec) if (e) (void)hipGraphExecDestroy(e);
+    for (auto g : b->graph) if (g) (void)hipGraphDestroy(g);
+    for (auto e : b->ev0) if (e) (void)hipEventDestroy(e);
+    for (auto e : b->ev1) if (e) (void)hipEventDestroy(e);
+    for (hipStream_t s : b->streams) (void)hipStreamDestroy(s);
+    if (b->slots) (void)hipHostFree(b->slots);
+    for (size_t i = 0; i < b->prev_cap.size() && i < b->ctx.size(); i++) b->ctx[i]->block_cap = b->prev_cap[i];
+    delete b;
+}
+
+extern "C" const char *mcl_batch_last_error(const mcl_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+// Capture one instance's step into a graph (its own stream in capture mode).
+static int batch_capture(mcl_batch *b, int i)
+{
+    mcl_ctx *c = b->ctx[i];
+    HostSlot *hs = b->slots + i;
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed) != hipSuccess)
+        return batch_fail(b, MCL_E_DEVICE, "instance %d: stream capture could not begin", i);
+    int rc = mcl_buoyancy_force(c);
+    if (!rc) rc = mcl_viscosity(c);
+    if (!rc && c->asm_blocks > 0) {
+        hipLaunchKernelGGL(k_assemble_K, dim3(c->asm_blocks), dim3(MCL_BLOCK), 0, c->stream,
+                           (const AsmLevel *)c->asm_tab, c->nlev);
+        if (hipGetLastError() != hipSuccess) rc = fail(c, MCL_E_DEVICE, "assembly launch");
+    }
+    // (the mixed path's FP32 hierarchy is rebuilt inside solve_enqueue)
+    if (!rc) rc = solve_enqueue(c, ctx_work(c), ENQ_GRAPH, nullptr, b->accuracy[i]);
+    if (!rc && hipMemcpyAsync(&hs->solve, c->stats, sizeof(DevStats), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        rc = fail(c, MCL_E_DEVICE, "stats copy");
+    if (!rc && hipMemcpyAsync(hs->resid, c->resid, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        rc = fail(c, MCL_E_DEVICE, "residual copy");
+    if (!rc) rc = heat_enqueue(c, ENQ_GRAPH, nullptr);
+    if (!rc && hipMemcpyAsync(&hs->heat, c->stats, sizeof(DevStats), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        rc = fail(c, MCL_E_DEVICE, "heat stats copy");
+    if (!rc && hipMemcpyAsync(&hs->model_time, c->model_time, sizeof(double), hipMemcpyDeviceToHost, c->stream) !=
+                   hipSuccess)
+        rc = fail(c, MCL_E_DEVICE, "model time copy");
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return batch_fail(b, rc, "instance %d: %s", i, c->err.c_str());
+    }
+    if (e != hipSuccess || !g) return batch_fail(b, MCL_E_DEVICE, "instance %d: capture: %s", i, hipGetErrorString(e));
+    b->graph[i] = g;
+    e = hipGraphInstantiate(&b->exec[i], g, nullptr, nullptr, 0);
+    if (e != hipSuccess) return batch_fail(b, MCL_E_DEVICE, "instance %d: instantiate: %s", i, hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int mcl_batch_create(mcl_ctx *const *ctxs, int n, int lanes, mcl_batch **out)
+{
+    if (!out) return MCL_E_ARG;
+    *out = nullptr;
+    if (!ctxs || n <= 0 || lanes <= 0) return MCL_E_ARG;
+    mcl_batch *b = new mcl_batch();
+    b->device = ctxs[0] ? ctxs[0]->device : 0;
+    for (int i = 0; i < n; i++) {
+        mcl_ctx *c = ctxs[i];
+        int rc = ready(c);
+        if (rc) { delete b; return rc; }
+        if (c->device != b->device) { delete b; return MCL_E_ARG; }
+        if (c->first_heat || !c->been_here_dp) {
+            // the first step (initial_T_dot, initial_d_p's first call) stays eager
+            c->err = "a batch starts after each instance's first step";
+            delete b;
+            return MCL_E_STATE;
+        }
+        if (c->ctl.auto_accuracy) {
+            c->err = "auto_accuracy needs a host read-back in every solve: not available in a batch";
+            delete b;
+            return MCL_E_UNSUPPORTED;
+        }
+        if (solve_checks(c)) { delete b; return MCL_E_ARG; }
+        b->ctx.push_back(c);
+    }
+    int rc = 0;
+    if (hipSetDevice(b->device) != hipSuccess) { delete b; return MCL_E_DEVICE; }
+    lanes = std::min(lanes, n);
+    b->lane.assign(n, 0);
+    b->graph.assign(n, nullptr);
+    b->exec.assign(n, nullptr);
+    b->ev0.assign(n, nullptr);
+    b->ev1.assign(n, nullptr);
+    b->accuracy.assign(n, 0.0);
+    for (mcl_ctx *c : b->ctx) b->prev_cap.push_back(c->block_cap);
+    // lanes: instances dealt largest first to the lane with the fewest rows
+    {
+        std::vector<int> order(n);
+        for (int i = 0; i < n; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            return b->ctx[x]->levels.back().neq > b->ctx[y]->levels.back().neq;
+        });
+        std::vector<long long> load(lanes, 0);
+        for (int i : order) {
+            int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            b->lane[i] = l;
+            load[l] += b->ctx[i]->levels.back().neq;
+        }
+    }
+    b->streams.assign(lanes, nullptr);
+    for (int l = 0; l < lanes && !rc; l++)
+        if (hipStreamCreateWithFlags(&b->streams[l], hipStreamNonBlocking) != hipSuccess)
+            rc = batch_fail(b, MCL_E_DEVICE, "lane stream");
+    if (!rc && hipHostMalloc((void **)&b->slots, n * sizeof(HostSlot), hipHostMallocDefault) != hipSuccess)
+        rc = batch_fail(b, MCL_E_DEVICE, "pinned statistics");
+    for (int i = 0; i < n && !rc; i++) {
+        mcl_ctx *c = b->ctx[i];
+        if (hipEventCreate(&b->ev0[i]) != hipSuccess || hipEventCreate(&b->ev1[i]) != hipSuccess) {
+            rc = batch_fail(b, MCL_E_DEVICE, "events");
+            break;
+        }
+        if (hipStreamSynchronize(c->stream) != hipSuccess) {
+            rc = batch_fail(b, MCL_E_DEVICE, "instance %d stream", i);
+            break;
+        }
+        const int share = std::max(1, c->max_coop_blocks / lanes);
+        c->block_cap = c->block_cap > 0 ? std::min(c->block_cap, share) : share;
+        b->accuracy[i] = c->ctl.accuracy;
+        // the mixed solve's halo and neighbour lists for the capped workgroup
+        // count are built now: their set-up allocates and may not be captured
+        if (c->ctl.precision == MCL_PREC_MIXED && c->have_mg) {
+            Level &T = c->levels.back();
+            const int nb = (int)coop_blocks(c, std::max(T.neq, T.nel), MCL_MIXED_ROWS_PER_BLOCK, "MCL_MIXED_BLOCKS");
+            if (nb != c->nbr_nb && build_neighbours(c, nb)) {
+                rc = batch_fail(b, MCL_E_DEVICE, "instance %d: %s", i, c->err.c_str());
+                break;
+            }
+        }
+        rc = batch_capture(b, i);
+    }
+    if (rc) {
+        int code = rc;
+        std::string msg = b->err;
+        mcl_batch_destroy(b);
+        (void)msg;
+        return code;
+    }
+    *out = b;
+    return 0;
+}
+
+// One time step of every instance whose skip[i] is 0 (skip may be NULL):
+// graph replays on the lanes, then a wait for all; statistics of instance i
+// in stokes[i], heat[i] and its step's device time (events around its graph
+// on its lane) in step_ms[i] (each output may be NULL).
+extern "C" int mcl_batch_step(mcl_batch *b, const int *skip, mcl_stokes_stats *stokes, mcl_heat_stats *heat,
+                              double *step_ms)
+{
+    if (!b) return MCL_E_ARG;
+    if (hipSetDevice(b->device) != hipSuccess) return batch_fail(b, MCL_E_DEVICE, "device");
+    const int n = (int)b->ctx.size();
+    for (int i = 0; i < n; i++) {
+        if (skip && skip[i]) continue;
+        hipStream_t s = b->streams[b->lane[i]];
+        if (hipEventRecord(b->ev0[i], s) != hipSuccess || hipGraphLaunch(b->exec[i], s) != hipSuccess ||
+            hipEventRecord(b->ev1[i], s) != hipSuccess)
+            return batch_fail(b, MCL_E_DEVICE, "instance %d: graph launch", i);
+    }
+    for (hipStream_t s : b->streams)
+        if (hipStreamSynchronize(s) != hipSuccess) return batch_fail(b, MCL_E_DEVICE, "lane synchronise");
+    int rc = 0;
+    for (int i = 0; i < n; i++) {
+        if (skip && skip[i]) continue;
+        mcl_ctx *c = b->ctx[i];
+        const HostSlot &hs = b->slots[i];
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, b->ev0[i], b->ev1[i]);
+        if (step_ms) step_ms[i] = ms;
+        if (hs.solve.aborted || hs.heat.aborted) {
+            (void)zero_async(c, c->gs.gen, MCL_SYNC_WORDS * sizeof(unsigned), false);
+            (void)hipStreamSynchronize(c->stream);
+            rc = batch_fail(b, MCL_E_DEVICE, "instance %d: grid barrier timed out", i);
+            continue;
+        }
+        solve_stats(c, hs.solve, hs.resid, b->accuracy[i], 0.f, stokes ? stokes + i : nullptr);
+        if (heat) {
+            heat[i].cg_cycles = hs.heat.cycles;
+            heat[i].init_cg_cycles = hs.heat.init_cycles;
+            heat[i].dt = hs.heat.dt;
+            heat[i].model_time = hs.model_time;
+            heat[i].device_ms = 0.0;
+        }
+        if (hs.solve.nan_flag && !rc) rc = batch_fail(b, MCL_E_NUMERIC, "instance %d: NaN in the Stokes solve", i);
+    }
+    return rc;
 }
 
 // ------------------------------------------------------------ learned guess
@@ -2297,7 +2592,7 @@ extern "C" int mcl_stokes_guess(mcl_ctx *c, mcl_net *net, int *accepted, double 
     HIPCHK(c, hipSetDevice(c->device));
     Level &L = c->levels.back();
     if (!c->mixed_ready) {
-        rc = mixed_prepare(c);
+        rc = mixed_prepare(c, false);
         if (rc) return rc;
     }
     if (!c->gd) {

@@ -791,15 +791,32 @@ __global__ void k_stress_node(LevelDev L, const double *__restrict__ es, const d
 
 // True residual norms after a solve: |F - K d - G p|^2, |G^T d|^2, |F|^2 as
 // partial sums in reduction slots 12..14 (k_collect3 finishes them).
-__global__ void k_residuals(LevelDev L, RedBuf rb, const double *__restrict__ d, const double *__restrict__ p)
+// Residual norms after a solve: |F - K d - G p|^2, |G^T d|^2 and |F|^2.
+// galerkin != 0 (the mixed path on meshes with hanging nodes, which solves
+// C^T K C u_f = C^T b): the momentum residual and F are measured in the
+// constrained space, C^T (F - K d - G p) and C^T F, rows of hanging
+// equations folded into their free parents.
+__global__ void k_residuals(LevelDev L, RedBuf rb, const double *__restrict__ d, const double *__restrict__ p,
+                            int galerkin)
 {
     __shared__ dd lds[3][MCL_BLOCK / MCL_WAVE];
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsize = gridDim.x * blockDim.x;
+    const bool gal = galerkin && L.nh > 0;
     dd acc[3] = {dd_zero(), dd_zero(), dd_zero()};
     for (int i = gtid; i < L.neq; i += gsize) {
+        if (gal && L.hp_n[i] != 0) continue;
         double r = L.F[i] - ell_row(L.Kcol, L.Kval, L.Kncol[i], L.neq, i, d) - g_row(L, i, p);
+        double f = L.F[i];
+        if (gal) {
+            for (int t = L.ch_ptr[i]; t < L.ch_ptr[i + 1]; t++) {
+                const int h = L.ch_i[t];
+                const double w = L.ch_w[t];
+                r += w * (L.F[h] - ell_row(L.Kcol, L.Kval, L.Kncol[h], L.neq, h, d) - g_row(L, h, p));
+                f += w * L.F[h];
+            }
+        }
         acc[0] = dd_add_prod(acc[0], r, r);
-        acc[2] = dd_add_prod(acc[2], L.F[i], L.F[i]);
+        acc[2] = dd_add_prod(acc[2], f, f);
     }
     for (int e = gtid; e < L.nel; e += gsize) {
         double c = gt_elem(L, e, d);

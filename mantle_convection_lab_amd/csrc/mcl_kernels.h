@@ -246,7 +246,7 @@ __global__ void k_uzawa_fp64(UzawaParams P);
 __global__ void k_v_to_nodes(LevelDev L, const double *d, double *V);
 __global__ void k_stress_elem(LevelDev L, const double *vis, const double *V, double *es, double *area);
 __global__ void k_stress_node(LevelDev L, const double *es, const double *area, double *stress);
-__global__ void k_residuals(LevelDev L, RedBuf rb, const double *d, const double *p);
+__global__ void k_residuals(LevelDev L, RedBuf rb, const double *d, const double *p, int galerkin);
 __global__ void k_collect3(RedBuf rb, int slot0, int nblocks, double *out);
 __global__ void k_heat_fp64(HeatParams H);
 __global__ void k_uzawa_mixed(MixedParams P);

@@ -7,6 +7,7 @@ device fields after converged solves of seeded random runs (Rayleigh number
 log-uniform in [1e5, 1e7], viscosity contrast log-uniform in [1, 1e4]).
 """
 import ctypes as C
+import json
 import math
 import os
 import time
@@ -155,6 +156,7 @@ def evaluate(weights, side, runs, steps, seed=7, log=print):
             m = Model(Input.from_keys(workload_keys(side, Ra, ctr, seed=s)), net=nn)
             recs = [m.step() for _ in range(steps)]
             res[label] = dict(uzawa=[x["uzawa"] for x in recs], solve_ms=[x["solve_ms"] for x in recs],
+                              step_wall_s=[x["wall_s"] for x in recs],
                               converged=[bool(x["converged"]) for x in recs],
                               accepted=[bool(x.get("guess_accepted", False)) for x in recs],
                               schur_res_net=[x.get("guess_res_net") for x in recs],
@@ -168,6 +170,28 @@ def evaluate(weights, side, runs, steps, seed=7, log=print):
     return out
 
 
+def summarize(ev):
+    """Totals over held-out runs: Uzawa iterations and wall time (the whole
+    step, network consultations included) with and without the network, per
+    step index and overall, and how often the gate took the network's guess."""
+    steps = len(ev[0]["net"]["uzawa"]) if ev else 0
+    out = dict(runs=len(ev), steps=steps, per_step=[])
+    for k in range(steps):
+        un = sum(r["net"]["uzawa"][k] for r in ev)
+        up = sum(r["plain"]["uzawa"][k] for r in ev)
+        wn = sum(r["net"]["step_wall_s"][k] for r in ev)
+        wp = sum(r["plain"]["step_wall_s"][k] for r in ev)
+        acc = sum(1 for r in ev if r["net"]["accepted"][k])
+        out["per_step"].append(dict(step=k, uzawa_net=un, uzawa_prev=up, wall_s_net=wn, wall_s_prev=wp,
+                                    net_accepted=acc))
+    out["uzawa_net"] = sum(p["uzawa_net"] for p in out["per_step"])
+    out["uzawa_prev"] = sum(p["uzawa_prev"] for p in out["per_step"])
+    out["wall_s_net"] = sum(p["wall_s_net"] for p in out["per_step"])
+    out["wall_s_prev"] = sum(p["wall_s_prev"] for p in out["per_step"])
+    out["all_converged"] = all(all(r[l]["converged"]) for r in ev for l in ("net", "plain"))
+    return out
+
+
 def main(argv=None):
     """Generate snapshots with the solver, train the U-Net on the GPU with the
     library's own kernels, save the weights, and evaluate on held-out runs.
@@ -176,7 +200,6 @@ def main(argv=None):
             --epochs 150 --batch 16 --out mantle_convection_lab_amd/weights/unet_256.npz
     """
     import argparse
-    import json
     ap = argparse.ArgumentParser()
     ap.add_argument("--side", type=int, default=256)
     ap.add_argument("--runs", type=int, default=200)
@@ -188,7 +211,17 @@ def main(argv=None):
     ap.add_argument("--out", default=None)
     ap.add_argument("--eval-runs", type=int, default=4)
     ap.add_argument("--eval-steps", type=int, default=3)
+    ap.add_argument("--eval-only", default=None, help="evaluate these weights on held-out runs; no training")
+    ap.add_argument("--eval-out", default=None, help="write the evaluation (JSON) to this file")
     args = ap.parse_args(argv)
+    if args.eval_only:
+        ev = evaluate(args.eval_only, args.side, args.eval_runs, args.eval_steps, seed=args.seed + 1000)
+        summ = summarize(ev)
+        print(json.dumps(summ), flush=True)
+        if args.eval_out:
+            with open(args.eval_out, "w") as f:
+                json.dump({"summary": summ, "runs": ev}, f)
+        return 0
     out = args.out or default_weights_path(args.side)
     net = Net(args.side, args.side, args.batch, seed=args.seed)
     t0 = time.perf_counter()

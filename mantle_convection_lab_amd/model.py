@@ -322,14 +322,15 @@ class Model:
     def remesh(self, min_depth, max_depth, refine_thr=0.05, coarsen_thr=0.01):
         """Adapt the quadtree to the current temperature (the mesh, the
         device context and its fields are rebuilt; T is transferred, the
-        solution restarts from zero; model time is kept).  When nothing is
-        refined or coarsened and the element list is unchanged, the current
-        mesh and context are kept, with their solution as the next warm
-        start."""
+        solution restarts from zero; model time is kept).  When the element
+        list comes out unchanged, the current mesh and context are kept, with
+        their solution as the next warm start."""
         T = self.field("T")
         t = self.ctx.model_time()
         new, Tn, nr, nc = self.mesh.adapt(T, min_depth, max_depth, refine_thr, coarsen_thr)
-        if nr == 0 and nc == 0 and np.array_equal(new.level(-1)["Mcode"], self.mesh.level(-1)["Mcode"]):
+        if np.array_equal(new.level(-1)["Mcode"], self.mesh.level(-1)["Mcode"]):
+            # the same element list (e.g. families coarsened and then refined
+            # again by the 2:1 balance): keep the mesh, context and warm start
             return 0, 0
         self.ctx.close()
         self.mesh = new

@@ -36,17 +36,21 @@ def test_fp64_bytes_per_iteration():
 
 
 def test_mixed_bytes_per_iteration():
-    # finest level only (no coarse levels): FP64 part nnz 12 + n_v 88, V-cycle
-    # 2 pre passes of nnz 8 + 3 n_v 4, first sweep 3 n_v 4, transfers 6 n_v 4
+    # finest level only (no coarse levels): FP64 part nnz 12 + n_v 88 and no
+    # V-cycle traffic (a single level is preconditioned by Jacobi)
     sizes = [(NV, NNZ, NP)]
-    pre = bench.MG_SWEEPS
     b = bench.mixed_bytes({"inner": 1, "uzawa": 0}, sizes)
-    want = NNZ * 12 + NV * 88 + 2 * pre * (NNZ * 8 + 3 * NV * 4) + 3 * NV * 4 + 6 * NV * 4
-    assert b == want
-    # the coarsest level (solved by its dense inverse) adds nothing; every
-    # level between it and the finest adds its own V-cycle traffic
+    fp64 = NNZ * 12 + NV * 88
+    assert b == fp64
+    # a level above the coarsest adds its K once per leg (two legs), 28 B of
+    # vectors per row and 8 B per row of the level below
     n2, z2 = 32768, 580000
     b2 = bench.mixed_bytes({"inner": 1, "uzawa": 0}, [(n2, z2, NP // 4), (NV, NNZ, NP)])
-    assert b2 == want
+    assert b2 == fp64 + 2 * NNZ * 8 + 28 * NV + 8 * n2
     b3 = bench.mixed_bytes({"inner": 1, "uzawa": 0}, [(8, 64, 4), (n2, z2, NP // 4), (NV, NNZ, NP)])
-    assert b3 == want + 2 * pre * (z2 * 8 + 3 * n2 * 4) + 3 * n2 * 4 + 6 * n2 * 4
+    assert b3 == b2 + 2 * z2 * 8 + 28 * n2 + 8 * 8
+    # 256x256 with its seven coarser levels: about 95 MB per inner iteration
+    # (39.7 MB FP64 + 55 MB V-cycle)
+    levels = [(NV // 4 ** k, NNZ // 4 ** k, NP // 4 ** k) for k in range(7, -1, -1)]
+    per = bench.mixed_bytes({"inner": 1, "uzawa": 0}, levels)
+    assert 93e6 < per < 97e6

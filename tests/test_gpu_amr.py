@@ -69,3 +69,29 @@ def test_unchanged_remesh_keeps_context():
     r2 = m.step()
     print(f"[gpu-amr] uzawa {r0['uzawa']}, {r1['uzawa']}, after the unchanged remesh {r2['uzawa']}", flush=True)
     assert r2["converged"] and r2["uzawa"] < r0["uzawa"]
+
+
+def test_c3_full_scale_with_hanging_nodes():
+    """The adaptive workload at its own size: aspect-2 box, base 128 x 64,
+    three extra levels, Ra = 1e7, viscosity contrast 1e4, mixed precision,
+    started from a developed temperature (boundary layers and 32 plumes, on
+    the base mesh adapted to it: about 0.31M velocity equations and 30k
+    hanging nodes) and remeshed once more after 10 steps.  Every solve must
+    converge with the momentum residual within 1e-6, the mesh must stay at
+    0.3-0.6M velocity equations with hanging nodes, and the final velocity
+    must satisfy the hanging-node constraint."""
+    from mantle_convection_lab_amd import amr
+    print("[gpu-amr] C3 full scale: base 128x64 +3 levels, Ra=1e7, contrast 1e4, 12 steps", flush=True)
+    out = amr.run(steps=12, every=10, base_nx=128, extra=3, developed=True, log=lambda s: print(s, flush=True),
+                  keep_model=True)
+    rs = out["recs"]
+    assert all(r["converged"] and r["res_mom"] <= 1e-6 for r in rs)
+    assert all(3e5 <= r["neq"] <= 6e5 and r["nh"] > 1000 for r in rs)
+    m = out["model"]
+    lv = m.mesh.level(-1)
+    V = m.field("V").reshape(2, -1)
+    h, c0, c1 = lv["h_ni"], lv["h_ci"][0], lv["h_ci"][1]
+    gap = np.abs(V[:, h] - 0.5 * (V[:, c0] + V[:, c1])).max()
+    print(f"  {lv['neq']} velocity equations, {lv['nh']} hanging nodes, constraint gap {gap:.2e}", flush=True)
+    assert gap <= 1e-12 * np.abs(V).max()
+    m.ctx.close()
