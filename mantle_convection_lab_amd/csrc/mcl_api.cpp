@@ -1795,6 +1795,11 @@ static int mixed_params(mcl_ctx *c, const UzawaParams &P, MixedParams &M)
     M.nc = c->levels[0].neq;
     M.rr = c->rr;
     M.inner_rtol = k.inner_rtol > 0 ? k.inner_rtol : 3e-7;
+    {
+        const char *e = getenv("MCL_INNER_RELAX"), *m = getenv("MCL_INNER_RELAX_MAX");
+        M.relax = e ? atof(e) : 0.0;
+        M.relax_max = m && atof(m) > 0 ? atof(m) : 1e-2;
+    }
     M.max_inner = 1000;
     M.probe = 0;
     M.use_nbr = 0;

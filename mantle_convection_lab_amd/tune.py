@@ -1,8 +1,8 @@
 """Launch-shape autotuner of the mixed-precision Stokes solve.
 
 For each mesh side it times the solve under every combination of
-  * MCL_COOP_BLOCKS  workgroups of the persistent solve (128, 192, 256 = one
-                     per CU of the MI355X),
+  * MCL_MIXED_BLOCKS workgroups of the persistent mixed solve (128, 192,
+                     256 = one per CU of the MI355X),
   * MCL_MG_FUSE      V-cycle phase fusion (1: one sweep per phase, 2: two),
   * MCL_MG_SWEEPS    Jacobi sweeps per level and leg (1, 2),
 each candidate in a fresh process (the library reads these settings when it
@@ -11,9 +11,12 @@ time steps of a seeded instance of that side (Ra=1e6, contrast 1e3; the same
 tolerance every time, so fewer sweeps may need more iterations).  The score is
 the summed device time of the solves.  The best candidate per side is written
 to weights/tuning.json; `apply(side)` exports it before a context is created
-(variables the caller has set itself are left alone).  On the MI355X the
-tuner picks 256 workgroups, two-sweep fusion and two sweeps at 128x128,
-256x256 and 512x512, the library's defaults there.
+(variables the caller has set itself are left alone).  The table is advisory:
+on the MI355X the tuner picks 256 workgroups, two-sweep fusion and two sweeps
+at 128x128, 256x256 and 512x512, which are the library's defaults there, so
+nothing applies it by default.  (The FP64 kernels' own knob, MCL_FP64_BLOCKS,
+was swept by hand: 129 workgroups at 256x256, the default, 32.6 us per inner
+iteration against 33.3-38.2 us for 192, 256 and 512.)
 
     python -m mantle_convection_lab_amd.tune [--sides 128 256 512] [--steps 3]
 """
@@ -24,7 +27,7 @@ import os
 import subprocess
 import sys
 
-KNOBS = ("MCL_COOP_BLOCKS", "MCL_MG_FUSE", "MCL_MG_SWEEPS")
+KNOBS = ("MCL_MIXED_BLOCKS", "MCL_MG_FUSE", "MCL_MG_SWEEPS")
 CANDIDATES = [dict(zip(KNOBS, v)) for v in itertools.product((128, 192, 256), (1, 2), (1, 2))]
 DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "tuning.json")
 

@@ -10,14 +10,14 @@ from mantle_convection_lab_amd import tune
 
 def test_apply_respects_caller_and_missing_entries(tmp_path, monkeypatch):
     path = tmp_path / "t.json"
-    path.write_text(json.dumps({"best": {"256": {"MCL_COOP_BLOCKS": 256, "MCL_MG_FUSE": 2, "MCL_MG_SWEEPS": 2}}}))
+    path.write_text(json.dumps({"best": {"256": {"MCL_MIXED_BLOCKS": 256, "MCL_MG_FUSE": 2, "MCL_MG_SWEEPS": 2}}}))
     for k in tune.KNOBS:
         monkeypatch.delenv(k, raising=False)
     monkeypatch.setenv("MCL_MG_SWEEPS", "1")
     done = tune.apply(256, str(path))
     try:
-        assert done == {"MCL_COOP_BLOCKS": 256, "MCL_MG_FUSE": 2}
-        assert os.environ["MCL_COOP_BLOCKS"] == "256" and os.environ["MCL_MG_SWEEPS"] == "1"
+        assert done == {"MCL_MIXED_BLOCKS": 256, "MCL_MG_FUSE": 2}
+        assert os.environ["MCL_MIXED_BLOCKS"] == "256" and os.environ["MCL_MG_SWEEPS"] == "1"
         assert tune.apply(64, str(path)) == {}
         assert tune.apply(256, str(tmp_path / "missing.json")) == {}
     finally:
