@@ -210,7 +210,8 @@ struct MixedParams {
     int nc;
     double *rr;              // FP64 residual of the inner CG
     double inner_rtol;       // relative residual tolerance of each inner solve
-    double relax, relax_max; // relax > 0: Uzawa-loop inner solves to min(relax_max, max(inner_rtol, relax / dv))
+    double relax, relax_max; // relax > 0: Uzawa-loop inner solves to min(relax_max, max(inner_rtol, relax / dv)),
+                             // then one correction solve of d for the final p
     int max_inner;
     int probe;               // 1: stop after the first inner solve and report |G^T d|^2
     // point-to-point waits in the V-cycle: neighbour workgroups of each

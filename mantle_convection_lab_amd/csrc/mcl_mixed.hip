@@ -1384,7 +1384,10 @@ __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
     if (P.relax > 0.0 && cycle > 0) {
         // final velocity correction for the converged pressure: d += K^-1
         // (F - G p - K d) to the first solve's accuracy, so the loosened
-        // inner solves leave no momentum defect (p and dv, dp are unchanged)
+        // inner solves leave no momentum defect (p and dv, dp are unchanged).
+        // d and p were written by their owning threads with plain stores and
+        // are now gathered across workgroups: a fenced barrier first.
+        ok = ok && grid_sync(U.gs, gen);
         for (int i = gtid; i < neq; i += gsize) {
             U.Ahv[i] = L.F[i] - g_row(L, i, U.p) - ell_row(L.Kcol, L.Kval, L.Kncol[i], neq, i, U.d);
             U.u1[i] = 0.0;
