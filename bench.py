@@ -129,21 +129,22 @@ def mixed_bytes(rec, sizes, pre=MG_SWEEPS):
     return rec["inner"] * (b_fp64 + b_v) + rec["uzawa"] * b_uz
 
 
-def pmc_traffic(kernel, recs):
+def pmc_traffic(kernel, recs, bytes_per_launch):
     """HBM bytes per launch of `kernel`: the committed rocprofv3 PMC summary
-    gives the traffic per inner iteration of a profiled launch (FETCH_SIZE +
-    WRITE_SIZE scaled by the copy-kernel factors); it is scaled to this run's
-    mean inner iterations per launch.  None without a summary."""
+    gives the measured traffic of a profiled launch (FETCH_SIZE + WRITE_SIZE
+    scaled by the copy-kernel factors) over that launch's algorithmic bytes;
+    the ratio is applied to this run's mean algorithmic bytes per launch.
+    None without a summary."""
     path = os.path.join(REPO, "profiles", f"{kernel}_256_pmc.txt")
     if not os.path.exists(path) or not recs:
         return None
-    per_inner = None
+    ratio = None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_inner_iteration:"):
-            per_inner = float(line.split(":", 1)[1])
-    if per_inner is None:
+        if line.startswith("traffic_over_algorithmic:"):
+            ratio = float(line.split(":", 1)[1])
+    if ratio is None:
         return None
-    return per_inner * sum(r["inner"] for r in recs) / len(recs)
+    return ratio * bytes_per_launch
 
 
 # ---------------------------------------------------------------- paths
@@ -285,7 +286,7 @@ def roofline(res, kernel):
     if not ms:
         return None
     achieved = sum(bytes_per) / (sum(ms) * 1e-3) / 1e9
-    traffic = pmc_traffic(kernel, recs)
+    traffic = pmc_traffic(kernel, recs, sum(bytes_per) / len(bytes_per))
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
             "avg_launch_ms": sum(ms) / len(ms), "bytes_per_launch": sum(bytes_per) / len(bytes_per)}
