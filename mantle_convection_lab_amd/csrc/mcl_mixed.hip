@@ -1027,18 +1027,22 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
     MCL_PHASE(1);
     const double stop2 = fmax(rtol * rtol * bn2, floor2);
     if (!(bn2 > stop2)) return 0;   // x = 0 (also on hanging equations)
-    if (use_mg) {
-        vcycle_rest(P, mgl, gs, gen, ok, nbr, r, 2, lds);
-        ok = csync(gs, gen, ok);
-        double o[1];
-        reduce_collect<1>(P.U.rb, 2, gridDim.x, o, lds);
-        delta = o[0];
-        MCL_PHASE(5);
-    }
-    double beta = 0.0;
+    // rz: the latest r.z (with Jacobi it came with the reduction above)
+    double beta = 0.0, rz = delta;
     bool first = true;
     int it = 0;
     while (ok && it < P.max_inner) {
+        // z = M r: the V-cycle's one call site, so the kernel holds one copy
+        if (use_mg) {
+            vcycle_rest(P, mgl, gs, gen, ok, nbr, r, 2, lds);
+            ok = csync(gs, gen, ok);
+            double o[1];
+            reduce_collect<1>(P.U.rb, 2, gridDim.x, o, lds);
+            rz = o[0];
+            MCL_PHASE(5);
+        }
+        if (!first) beta = rz / delta;
+        delta = rz;
         double dq;
         {
             dd acc[1] = {dd_zero()};
@@ -1133,7 +1137,8 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
         }
         if (!(dq > 0.0)) break;
         const double alpha = delta / dq;
-        double rr, rz = 0.0;
+        double rr;
+        rz = 0.0;
         {
             dd acc[2] = {dd_zero(), dd_zero()};
             for (int i = ex.b; i < ex.e; i += ex.s) {
@@ -1161,16 +1166,6 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
         }
         it++;
         if (rr <= stop2) break;
-        if (use_mg) {
-            vcycle_rest(P, mgl, gs, gen, ok, nbr, r, 2, lds);
-            ok = csync(gs, gen, ok);
-            double o[1];
-            reduce_collect<1>(P.U.rb, 2, gridDim.x, o, lds);
-            rz = o[0];
-            MCL_PHASE(5);
-        }
-        beta = rz / delta;
-        delta = rz;
         first = false;
         MCL_G double *t = dold;
         dold = dnew;
@@ -1239,165 +1234,163 @@ __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
         }
         U.BAB[e] = s;
     }
-    // momentum defect of the initial guess: Ah = F - G p - K d; |F|^2 gives
-    // the first inner solve an absolute floor inner_rtol |F| (a warm start's
-    // small defect is not reduced by another factor inner_rtol)
-    {
-        dd acc[1] = {dd_zero()};
-        for (int i = gtid; i < neq; i += gsize) {
-            double u = ell_row(L.Kcol, L.Kval, L.Kncol[i], neq, i, U.d);
-            double f = L.F[i];
-            U.Ahv[i] = f - g_row(L, i, U.p) - u;
-            U.u1[i] = 0.0;
-            acc[0] = dd_add_prod(acc[0], f, f);
-        }
-        reduce_publish<1>(U.rb, 5, acc, lds);
-    }
-    spmv++;
-    ok = ok && grid_sync(U.gs, gen);
-    double ff;
-    {
-        double o[1];
-        reduce_collect<1>(U.rb, 5, gridDim.x, o, lds);
-        ff = o[0];
-    }
-    MCL_PHASE(6);
-    inner += inner_mixed(P, U.Ahv, U.u1, true, U.gs, gen, ok, lds, mgl, spmv,
-                         P.inner_rtol * P.inner_rtol * ff, nbr, P.inner_rtol);
-    for (int i = gtid; i < neq; i += gsize) U.d[i] += U.u1[i];
-    ok = ok && grid_sync(U.gs, gen);
+    // One inner solve call site for all three kinds of inner solve (the
+    // kernel then holds one copy of the inner solve and its V-cycle):
+    //   stage 0: the first solve, for the momentum defect of the initial guess;
+    //   stage 1: the Uzawa iterations' solves K u1 = G s2;
+    //   stage 2: with relaxed inner accuracy, the final velocity correction.
     double *r0 = U.r0, *r1 = U.r1, *r2 = U.r2, *z0 = U.z0, *z1 = U.z1, *s1 = U.s1, *s2 = U.s2;
-    if (P.probe) {
-        // initial Schur residual r1 = G^T K^-1 (F - G p) of this (d, p): the
-        // quantity the Uzawa iteration has to reduce
-        dd acc[1] = {dd_zero()};
-        for (int e = gtid; e < nel; e += gsize) {
-            double r = gt_elem(L, e, U.d);
-            acc[0] = dd_add_prod(acc[0], r, r);
-        }
-        reduce_publish<1>(U.rb, 4, acc, lds);
-        ok = ok && grid_sync(U.gs, gen);
-        double o[1];
-        reduce_collect<1>(U.rb, 4, gridDim.x, o, lds);
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            U.stats->probe_r1 = o[0];
-            U.stats->cycles = 0;
-            U.stats->inner = inner;
-            U.stats->spmv = spmv;
-            U.stats->aborted = ok ? 0 : 1;
-        }
-        return;
-    }
-    // Uzawa iterations: every vector here is read only by the thread that
-    // writes it (element and equation loops use the same grid stride),
-    // except s2, which G s2 gathers across workgroups (agent-scope accesses)
-    // and Ahv, which the inner solve reads on its own rows (a fenced barrier);
-    // the inner solve ends with a fenced barrier, so u1 is read plainly.  The
-    // other waits need no cache maintenance.
-    for (int e = gtid; e < nel; e += gsize) r1[e] = gt_elem(L, e, U.d);
-    double dv = 1.0, dp = 1.0;
-    int cycle = 0;
-    while (ok && (dv > U.citcom_accu || (dp > U.citcom_accu && U.check_v_only == 0)) &&
-           cycle < U.max_citcom_cycle) {
-        {
-            dd acc[2] = {dd_zero(), dd_zero()};
-            for (int e = gtid; e < nel; e += gsize) {
-                double z = U.schur ? r1[e] / U.BAB[e] : U.BAB[e] * r1[e];
-                z1[e] = z;
-                acc[0] = dd_add_prod(acc[0], r1[e], z);
-                if (cycle > 0) acc[1] = dd_add_prod(acc[1], r0[e], z0[e]);
-            }
-            reduce_publish<2>(U.rb, 4, acc, lds);
-        }
-        ok = csync(U.gs, gen, ok);
-        double r1z1, delta = 0.0;
-        {
-            double o[2];
-            reduce_collect<2>(U.rb, 4, gridDim.x, o, lds);
-            r1z1 = o[0];
-            if (cycle > 0) delta = r1z1 / o[1];
-        }
-        for (int e = gtid; e < nel; e += gsize)
-            vstd<kCO>(s2 + e, cycle == 0 ? z1[e] : z1[e] + delta * vldd<kCO>(s1 + e));
-        ok = csync(U.gs, gen, ok);
-        for (int i = gtid; i < neq; i += gsize) {
-            U.Ahv[i] = g_row_ld(L, i, [&](int e) { return vldd<kCO>(s2 + e); });
-            U.u1[i] = 0.0;
-        }
-        ok = ok && grid_sync(U.gs, gen);   // the inner solve reads Ahv on its own rows
-        MCL_PHASE(6);
-        // relaxed inner accuracy (P.relax > 0): the update this solve feeds
-        // is about dv |d| in size, so its defect matters in proportion; the
-        // tolerance grows as relax / dv up to relax_max, and a final
-        // correction below restores the momentum residual
-        double rtol = P.inner_rtol;
-        if (P.relax > 0.0) rtol = fmin(P.relax_max, fmax(P.inner_rtol, P.relax / fmax(dv, 1e-300)));
-        inner += inner_mixed(P, U.Ahv, U.u1, true, U.gs, gen, ok, lds, mgl, spmv, 0.0, nbr, rtol);
-        {
+    double dv = 1.0, dp = 1.0, ff = 0.0, r1z1 = 0.0;
+    int cycle = 0, stage = 0;
+    while (ok) {
+        double floor2 = 0.0, rtol = P.inner_rtol;
+        if (stage != 1) {
+            // momentum defect Ah = F - G p - K d; |F|^2 gives the solve an
+            // absolute floor inner_rtol |F| (a warm start's small defect is
+            // not reduced by another factor inner_rtol).  After the Uzawa
+            // iterations d and p were written by their owning threads with
+            // plain stores and are now gathered across workgroups: a fenced
+            // barrier first.
+            if (stage == 2) ok = ok && grid_sync(U.gs, gen);
             dd acc[1] = {dd_zero()};
-            for (int e = gtid; e < nel; e += gsize) {
-                double a = gt_elem(L, e, U.u1);
-                U.Ahp[e] = a;
-                acc[0] = dd_add_prod(acc[0], vldd<kCO>(s2 + e), a);
-            }
-            reduce_publish<1>(U.rb, 6, acc, lds);
-        }
-        ok = csync(U.gs, gen, ok);
-        double alpha;
-        {
-            double o[1];
-            reduce_collect<1>(U.rb, 6, gridDim.x, o, lds);
-            alpha = r1z1 / o[0];
-        }
-        {
-            dd acc[4] = {dd_zero(), dd_zero(), dd_zero(), dd_zero()};
-            for (int e = gtid; e < nel; e += gsize) {
-                r2[e] = r1[e] - alpha * U.Ahp[e];
-                const double se = vldd<kCO>(s2 + e);
-                double pe = U.p[e] + alpha * se;
-                U.p[e] = pe;
-                acc[0] = dd_add_prod(acc[0], se, se);
-                acc[1] = dd_add_prod(acc[1], pe, pe);
-            }
             for (int i = gtid; i < neq; i += gsize) {
-                double u = U.u1[i];
-                double di = U.d[i] - alpha * u;
-                U.d[i] = di;
-                acc[2] = dd_add_prod(acc[2], u, u);
-                acc[3] = dd_add_prod(acc[3], di, di);
+                double u = ell_row(L.Kcol, L.Kval, L.Kncol[i], neq, i, U.d);
+                double f = L.F[i];
+                U.Ahv[i] = f - g_row(L, i, U.p) - u;
+                U.u1[i] = 0.0;
+                acc[0] = dd_add_prod(acc[0], f, f);
             }
-            reduce_publish<4>(U.rb, 8, acc, lds);
+            reduce_publish<1>(U.rb, 5, acc, lds);
+            spmv++;
+            ok = ok && grid_sync(U.gs, gen);
+            double o[1];
+            reduce_collect<1>(U.rb, 5, gridDim.x, o, lds);
+            ff = o[0];
+            floor2 = P.inner_rtol * P.inner_rtol * ff;
+        } else {
+            // Uzawa iterations: every vector here is read only by the thread
+            // that writes it (element and equation loops use the same grid
+            // stride), except s2, which G s2 gathers across workgroups
+            // (agent-scope accesses) and Ahv, which the inner solve reads on
+            // its own rows (a fenced barrier); the inner solve ends with a
+            // fenced barrier, so u1 is read plainly.  The other waits need no
+            // cache maintenance.
+            {
+                dd acc[2] = {dd_zero(), dd_zero()};
+                for (int e = gtid; e < nel; e += gsize) {
+                    double z = U.schur ? r1[e] / U.BAB[e] : U.BAB[e] * r1[e];
+                    z1[e] = z;
+                    acc[0] = dd_add_prod(acc[0], r1[e], z);
+                    if (cycle > 0) acc[1] = dd_add_prod(acc[1], r0[e], z0[e]);
+                }
+                reduce_publish<2>(U.rb, 4, acc, lds);
+            }
+            ok = csync(U.gs, gen, ok);
+            double delta = 0.0;
+            {
+                double o[2];
+                reduce_collect<2>(U.rb, 4, gridDim.x, o, lds);
+                r1z1 = o[0];
+                if (cycle > 0) delta = r1z1 / o[1];
+            }
+            for (int e = gtid; e < nel; e += gsize)
+                vstd<kCO>(s2 + e, cycle == 0 ? z1[e] : z1[e] + delta * vldd<kCO>(s1 + e));
+            ok = csync(U.gs, gen, ok);
+            for (int i = gtid; i < neq; i += gsize) {
+                U.Ahv[i] = g_row_ld(L, i, [&](int e) { return vldd<kCO>(s2 + e); });
+                U.u1[i] = 0.0;
+            }
+            ok = ok && grid_sync(U.gs, gen);   // the inner solve reads Ahv on its own rows
+            // relaxed inner accuracy (P.relax > 0): the update this solve
+            // feeds is about dv |d| in size, so its defect matters in
+            // proportion; the tolerance grows as relax / dv up to relax_max,
+            // and the final correction (stage 2) restores the momentum residual
+            if (P.relax > 0.0) rtol = fmin(P.relax_max, fmax(P.inner_rtol, P.relax / fmax(dv, 1e-300)));
         }
-        ok = csync(U.gs, gen, ok);
-        {
-            double o[4];
-            reduce_collect<4>(U.rb, 8, gridDim.x, o, lds);
-            dp = alpha * sqrt(o[0] / (1.0e-32 + o[1]));
-            dv = alpha * sqrt(o[2] / (1.0e-32 + o[3]));
+        MCL_PHASE(6);
+        inner += inner_mixed(P, U.Ahv, U.u1, true, U.gs, gen, ok, lds, mgl, spmv, floor2, nbr, rtol);
+        if (stage != 1) {
+            for (int i = gtid; i < neq; i += gsize) U.d[i] += U.u1[i];
+            ok = ok && grid_sync(U.gs, gen);
+            if (stage == 2) break;
+            if (P.probe) {
+                // initial Schur residual r1 = G^T K^-1 (F - G p) of this
+                // (d, p): the quantity the Uzawa iteration has to reduce
+                dd acc[1] = {dd_zero()};
+                for (int e = gtid; e < nel; e += gsize) {
+                    double r = gt_elem(L, e, U.d);
+                    acc[0] = dd_add_prod(acc[0], r, r);
+                }
+                reduce_publish<1>(U.rb, 4, acc, lds);
+                ok = ok && grid_sync(U.gs, gen);
+                double o[1];
+                reduce_collect<1>(U.rb, 4, gridDim.x, o, lds);
+                if (blockIdx.x == 0 && threadIdx.x == 0) {
+                    U.stats->probe_r1 = o[0];
+                    U.stats->cycles = 0;
+                    U.stats->inner = inner;
+                    U.stats->spmv = spmv;
+                    U.stats->aborted = ok ? 0 : 1;
+                }
+                return;
+            }
+            for (int e = gtid; e < nel; e += gsize) r1[e] = gt_elem(L, e, U.d);
+            stage = 1;
+        } else {
+            {
+                dd acc[1] = {dd_zero()};
+                for (int e = gtid; e < nel; e += gsize) {
+                    double a = gt_elem(L, e, U.u1);
+                    U.Ahp[e] = a;
+                    acc[0] = dd_add_prod(acc[0], vldd<kCO>(s2 + e), a);
+                }
+                reduce_publish<1>(U.rb, 6, acc, lds);
+            }
+            ok = csync(U.gs, gen, ok);
+            double alpha;
+            {
+                double o[1];
+                reduce_collect<1>(U.rb, 6, gridDim.x, o, lds);
+                alpha = r1z1 / o[0];
+            }
+            {
+                dd acc[4] = {dd_zero(), dd_zero(), dd_zero(), dd_zero()};
+                for (int e = gtid; e < nel; e += gsize) {
+                    r2[e] = r1[e] - alpha * U.Ahp[e];
+                    const double se = vldd<kCO>(s2 + e);
+                    double pe = U.p[e] + alpha * se;
+                    U.p[e] = pe;
+                    acc[0] = dd_add_prod(acc[0], se, se);
+                    acc[1] = dd_add_prod(acc[1], pe, pe);
+                }
+                for (int i = gtid; i < neq; i += gsize) {
+                    double u = U.u1[i];
+                    double di = U.d[i] - alpha * u;
+                    U.d[i] = di;
+                    acc[2] = dd_add_prod(acc[2], u, u);
+                    acc[3] = dd_add_prod(acc[3], di, di);
+                }
+                reduce_publish<4>(U.rb, 8, acc, lds);
+            }
+            ok = csync(U.gs, gen, ok);
+            {
+                double o[4];
+                reduce_collect<4>(U.rb, 8, gridDim.x, o, lds);
+                dp = alpha * sqrt(o[0] / (1.0e-32 + o[1]));
+                dv = alpha * sqrt(o[2] / (1.0e-32 + o[3]));
+            }
+            cycle++;
+            double *sh;
+            sh = s1; s1 = s2; s2 = sh;
+            sh = r0; r0 = r1; r1 = r2; r2 = sh;
+            sh = z0; z0 = z1; z1 = sh;
         }
-        cycle++;
-        double *sh;
-        sh = s1; s1 = s2; s2 = sh;
-        sh = r0; r0 = r1; r1 = r2; r2 = sh;
-        sh = z0; z0 = z1; z1 = sh;
-    }
-    if (P.relax > 0.0 && cycle > 0) {
-        // final velocity correction for the converged pressure: d += K^-1
-        // (F - G p - K d) to the first solve's accuracy, so the loosened
-        // inner solves leave no momentum defect (p and dv, dp are unchanged).
-        // d and p were written by their owning threads with plain stores and
-        // are now gathered across workgroups: a fenced barrier first.
-        ok = ok && grid_sync(U.gs, gen);
-        for (int i = gtid; i < neq; i += gsize) {
-            U.Ahv[i] = L.F[i] - g_row(L, i, U.p) - ell_row(L.Kcol, L.Kval, L.Kncol[i], neq, i, U.d);
-            U.u1[i] = 0.0;
+        // the Uzawa loop's test (Solver.c:131), then the final correction
+        if (!((dv > U.citcom_accu || (dp > U.citcom_accu && U.check_v_only == 0)) &&
+              cycle < U.max_citcom_cycle)) {
+            if (P.relax > 0.0 && cycle > 0) stage = 2;
+            else break;
         }
-        spmv++;
-        ok = ok && grid_sync(U.gs, gen);
-        inner += inner_mixed(P, U.Ahv, U.u1, true, U.gs, gen, ok, lds, mgl, spmv,
-                             P.inner_rtol * P.inner_rtol * ff, nbr, P.inner_rtol);
-        for (int i = gtid; i < neq; i += gsize) U.d[i] += U.u1[i];
-        ok = ok && grid_sync(U.gs, gen);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         U.stats->cycles = cycle;
