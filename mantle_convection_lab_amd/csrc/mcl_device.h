@@ -219,10 +219,15 @@ __device__ __forceinline__ double dmax_shfl(double v)
     return v;
 }
 
+// Rows of a kernel's reduction scratch (__shared__ dd lds[MCL_RED_ROWS]
+// [MCL_BLOCK / MCL_WAVE]): a reduction of NR values uses rows 0 .. NR - 1.
+#define MCL_RED_ROWS 8
+
 // Block reduction of NR double-double values (fixed order, deterministic).
 template <int NR>
 __device__ __forceinline__ void block_reduce_dd(dd (&v)[NR], dd (*lds)[MCL_BLOCK / MCL_WAVE])
 {
+    static_assert(NR <= MCL_RED_ROWS, "the reduction scratch has MCL_RED_ROWS rows");
     const int lane = threadIdx.x & (MCL_WAVE - 1), wave = threadIdx.x / MCL_WAVE;
 #pragma unroll
     for (int r = 0; r < NR; r++)

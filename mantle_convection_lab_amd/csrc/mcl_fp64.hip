@@ -585,7 +585,7 @@ __device__ int inner_solve(const UzawaParams &P, const LevelDev &L, const EllMat
 // citcom_solver (Solver.c:14-249) as one persistent kernel.
 __global__ void __launch_bounds__(MCL_BLOCK) k_uzawa_fp64(UzawaParams P)
 {
-    __shared__ dd lds[4][MCL_BLOCK / MCL_WAVE];
+    __shared__ dd lds[MCL_RED_ROWS][MCL_BLOCK / MCL_WAVE];
     __shared__ double ldsm[MCL_BLOCK / MCL_WAVE];
     const LevelDev &L = P.L;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsize = gridDim.x * blockDim.x;
@@ -799,7 +799,7 @@ __global__ void k_stress_node(LevelDev L, const double *__restrict__ es, const d
 __global__ void k_residuals(LevelDev L, RedBuf rb, const double *__restrict__ d, const double *__restrict__ p,
                             int galerkin)
 {
-    __shared__ dd lds[3][MCL_BLOCK / MCL_WAVE];
+    __shared__ dd lds[MCL_RED_ROWS][MCL_BLOCK / MCL_WAVE];
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsize = gridDim.x * blockDim.x;
     const bool gal = galerkin && L.nh > 0;
     dd acc[3] = {dd_zero(), dd_zero(), dd_zero()};
@@ -827,7 +827,7 @@ __global__ void k_residuals(LevelDev L, RedBuf rb, const double *__restrict__ d,
 
 __global__ void k_collect3(RedBuf rb, int slot0, int nblocks, double *out)
 {
-    __shared__ dd lds[3][MCL_BLOCK / MCL_WAVE];
+    __shared__ dd lds[MCL_RED_ROWS][MCL_BLOCK / MCL_WAVE];
     double o[3];
     reduce_collect<3>(rb, slot0, nblocks, o, lds);
     if (threadIdx.x == 0)
@@ -926,7 +926,7 @@ __device__ void heat_assemble_row(const LevelDev &L, const HeatParams &H, int P)
 // update_temperature (Update_temperature.c:3-133) as one persistent kernel.
 __global__ void __launch_bounds__(MCL_BLOCK) k_heat_fp64(HeatParams H)
 {
-    __shared__ dd lds[4][MCL_BLOCK / MCL_WAVE];
+    __shared__ dd lds[MCL_RED_ROWS][MCL_BLOCK / MCL_WAVE];
     __shared__ double ldsm[MCL_BLOCK / MCL_WAVE];
     const LevelDev &L = H.L;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsize = gridDim.x * blockDim.x;

@@ -960,23 +960,29 @@ __device__ __forceinline__ double ct_gather(const LevelDev &L, int i, double yi,
 }
 
 // ------------------------------------------------- FP64 CG, FP32 V-cycle
-// Inner solve K x = b (x = 0 on entry when x_zero) by FP64 conjugate gradients
-// preconditioned with the FP32 multigrid V-cycle (Jacobi when there is a
-// single level): K x products, residuals and recurrences stay in FP64 with
-// compensated dot products; only z = M r is FP32.  Stops at
-// |b - K x|^2 <= max(rtol^2 |b|^2, floor2).  Per iteration:
+// Inner solve K x = b from x = 0 by FP64 conjugate gradients preconditioned
+// with the FP32 multigrid V-cycle (Jacobi when there is a single level): K x
+// products, residuals and recurrences stay in FP64 with compensated dot
+// products; only z = M r is FP32.  Stops at |b - K x|^2 <= max(rtol^2 |b|^2,
+// floor2), floor2 = ff_coef ff.  Per iteration:
 //   (1) d' = z + beta d and q = K d' in one pass (q gathers z + beta d),
 //       with d'.q                                              | barrier
 //   (2) x += alpha d', r -= alpha q, r.r, finest b = r and the
 //       first Jacobi sweep                                     | barrier
 //   (3) rest of the V-cycle, last sweep fused with r.z          | barrier
-// Reduction slots: 0,1 (start), 2 (r.z), 12 (d.q), 13,14 (r.r, r.z).
+// The caller writes b on this thread's rows (own_rows(neq)) in the phase
+// that calls the solve, so the start needs no barrier before it (with
+// hanging nodes the start reads b across rows: the caller waits on the grid
+// first).  With collect_ff the caller has published |F|^2 to slot 5 in that
+// phase and ff receives it.  x is stored with agent-scope stores and is
+// complete for agent-scope loads on return, which follows a barrier (with
+// hanging nodes, a fenced one after the expansion u = C u_f).
+// Reduction slots: 0,1 (start), 2 (r.z), 15 (d.q), 13,14 (r.r, r.z).
 // Returns the iteration count.
 __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *__restrict__ b,
-                                           double *__restrict__ x, bool x_zero, const GridSync &gs,
-                                           unsigned &gen, bool &ok, dd (*lds)[MCL_BLOCK / MCL_WAVE],
-                                           const MgLds &mgl, long long &spmv, double floor2, int nbr,
-                                           double rtol)
+                                           double *__restrict__ x, const GridSync &gs, unsigned &gen, bool &ok,
+                                           dd (*lds)[MCL_BLOCK / MCL_WAVE], const MgLds &mgl, double ff_coef,
+                                           double &ff, bool collect_ff, int nbr, double rtol)
 {
     const LevelDev &L = P.U.L;
     const bool use_mg = P.nmg > 1;
@@ -991,12 +997,13 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
     const Ex ex = own_rows(n);
     double *__restrict__ r = P.rr, *__restrict__ q = P.U.cg_q;
     MCL_G double *dold = gp(P.U.cg_d), *dnew = gp(P.U.cg_s);
-    // r = b - K x, then M r
+    MCL_G double *xg = gp(x);
+    // r = b, then M r
     {
         dd acc[2] = {dd_zero(), dd_zero()};
         for (int i = ex.b; i < ex.e; i += ex.s) {
-            double ri = x_zero ? b[i] : b[i] - ell_row(L.Kcol, L.Kval, L.Kncol[i], n, i, x);
-            if (x_zero) x[i] = 0.0;
+            double ri = b[i];
+            vstd<kCO>(xg + i, 0.0);
             if (L.nh > 0) {
                 // r = C^T b (the solve starts from x = 0 whenever there are hanging nodes)
                 ri = L.hp_n[i] != 0 ? 0.0 : ct_gather(L, i, b[i], [&](int h) { return b[h]; });
@@ -1013,19 +1020,23 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
                 acc[1] = dd_add_prod(acc[1], ri, (double)zi);
             }
         }
-        if (!x_zero) spmv++;
         reduce_publish<2>(P.U.rb, 0, acc, lds);
     }
-    ok = ok && grid_sync(gs, gen);
+    ok = csync(gs, gen, ok);
     double bn2, delta;
     {
         double o[2];
         reduce_collect<2>(P.U.rb, 0, gridDim.x, o, lds);
         bn2 = o[0];
         delta = o[1];
+        if (collect_ff) {
+            double f[1];
+            reduce_collect<1>(P.U.rb, 5, gridDim.x, f, lds);
+            ff = f[0];
+        }
     }
     MCL_PHASE(1);
-    const double stop2 = fmax(rtol * rtol * bn2, floor2);
+    const double stop2 = fmax(rtol * rtol * bn2, ff_coef * ff);
     if (!(bn2 > stop2)) return 0;   // x = 0 (also on hanging equations)
     // rz: the latest r.z (with Jacobi it came with the reduction above)
     double beta = 0.0, rz = delta;
@@ -1127,11 +1138,10 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
                 q[i] = s;
                 acc[0] = dd_add_prod(acc[0], di, s);
             }
-            spmv++;
-            reduce_publish<1>(P.U.rb, 12, acc, lds);
+            reduce_publish<1>(P.U.rb, 15, acc, lds);
             ok = csync(gs, gen, ok);
             double o[1];
-            reduce_collect<1>(P.U.rb, 12, gridDim.x, o, lds);
+            reduce_collect<1>(P.U.rb, 15, gridDim.x, o, lds);
             dq = o[0];
             MCL_PHASE(0);
         }
@@ -1142,7 +1152,7 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
         {
             dd acc[2] = {dd_zero(), dd_zero()};
             for (int i = ex.b; i < ex.e; i += ex.s) {
-                x[i] += alpha * vldd<kCO>(dnew + i);
+                vstd<kCO>(xg + i, vldd<kCO>(xg + i) + alpha * vldd<kCO>(dnew + i));
                 double ri = r[i] - alpha * q[i];
                 r[i] = ri;
                 acc[0] = dd_add_prod(acc[0], ri, ri);
@@ -1171,12 +1181,12 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
         dold = dnew;
         dnew = t;
     }
-    // x was written with plain stores since the last fenced barrier
-    ok = ok && grid_sync(gs, gen);
+    // every exit follows a barrier that ends the phase of the last x update
     if (L.nh > 0) {
-        // u = C u_f
+        // u = C u_f (hanging values from their parents across rows)
+        ok = ok && grid_sync(gs, gen);
         for (int i = ex.b; i < ex.e; i += ex.s)
-            if (L.hp_n[i] != 0) x[i] = c_expand(L, i, [&](int j) { return x[j]; });
+            if (L.hp_n[i] != 0) vstd<kCO>(xg + i, c_expand(L, i, [&](int j) { return vldd<kCO>(xg + j); }));
         ok = ok && grid_sync(gs, gen);
     }
     return it;
@@ -1186,7 +1196,7 @@ __device__ __forceinline__ int inner_mixed(const MixedParams &P, const double *_
 
 __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
 {
-    __shared__ dd lds[4][MCL_BLOCK / MCL_WAVE];
+    __shared__ dd lds[MCL_RED_ROWS][MCL_BLOCK / MCL_WAVE];
     __shared__ float mg_vec[3 * MCL_MG_LDS_ROWS], mg_dinv[MCL_MG_LDS_ROWS], mg_val[MCL_MG_LDS_NNZ];
     __shared__ unsigned short mg_col[MCL_MG_LDS_NNZ];
     // level descriptors and Gershgorin bounds (read in every phase; LDS keeps
@@ -1239,11 +1249,20 @@ __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
     //   stage 0: the first solve, for the momentum defect of the initial guess;
     //   stage 1: the Uzawa iterations' solves K u1 = G s2;
     //   stage 2: with relaxed inner accuracy, the final velocity correction.
+    // The inner solve's right-hand side is written on the rows the solve's
+    // threads own (own_rows), in the phase that starts the solve.  Element
+    // vectors are read by the thread that writes them (the element loops use
+    // the grid stride), except z and s, which G gathers across workgroups,
+    // and u1, which the solve stores and G^T gathers: those three use
+    // agent-scope accesses, and the waits between their phases need no cache
+    // maintenance.
     double *r0 = U.r0, *r1 = U.r1, *r2 = U.r2, *z0 = U.z0, *z1 = U.z1, *s1 = U.s1, *s2 = U.s2;
-    double dv = 1.0, dp = 1.0, ff = 0.0, r1z1 = 0.0;
+    MCL_G double *u1 = gp(U.u1);
+    const Ex rows = own_rows(neq);
+    double dv = 1.0, dp = 1.0, ff = 0.0, r1z1 = 0.0, r0z0 = 0.0;
     int cycle = 0, stage = 0;
     while (ok) {
-        double floor2 = 0.0, rtol = P.inner_rtol;
+        double rtol = P.inner_rtol, ffc = 0.0;
         if (stage != 1) {
             // momentum defect Ah = F - G p - K d; |F|^2 gives the solve an
             // absolute floor inner_rtol |F| (a warm start's small defect is
@@ -1253,54 +1272,29 @@ __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
             // barrier first.
             if (stage == 2) ok = ok && grid_sync(U.gs, gen);
             dd acc[1] = {dd_zero()};
-            for (int i = gtid; i < neq; i += gsize) {
+            for (int i = rows.b; i < rows.e; i += rows.s) {
                 double u = ell_row(L.Kcol, L.Kval, L.Kncol[i], neq, i, U.d);
                 double f = L.F[i];
                 U.Ahv[i] = f - g_row(L, i, U.p) - u;
-                U.u1[i] = 0.0;
                 acc[0] = dd_add_prod(acc[0], f, f);
             }
-            reduce_publish<1>(U.rb, 5, acc, lds);
             spmv++;
-            ok = ok && grid_sync(U.gs, gen);
-            double o[1];
-            reduce_collect<1>(U.rb, 5, gridDim.x, o, lds);
-            ff = o[0];
-            floor2 = P.inner_rtol * P.inner_rtol * ff;
+            if (stage == 0) reduce_publish<1>(U.rb, 5, acc, lds);
+            ffc = P.inner_rtol * P.inner_rtol;
         } else {
-            // Uzawa iterations: every vector here is read only by the thread
-            // that writes it (element and equation loops use the same grid
-            // stride), except s2, which G s2 gathers across workgroups
-            // (agent-scope accesses) and Ahv, which the inner solve reads on
-            // its own rows (a fenced barrier); the inner solve ends with a
-            // fenced barrier, so u1 is read plainly.  The other waits need no
-            // cache maintenance.
-            {
-                dd acc[2] = {dd_zero(), dd_zero()};
-                for (int e = gtid; e < nel; e += gsize) {
-                    double z = U.schur ? r1[e] / U.BAB[e] : U.BAB[e] * r1[e];
-                    z1[e] = z;
-                    acc[0] = dd_add_prod(acc[0], r1[e], z);
-                    if (cycle > 0) acc[1] = dd_add_prod(acc[1], r0[e], z0[e]);
-                }
-                reduce_publish<2>(U.rb, 4, acc, lds);
+            // Uzawa iteration `cycle`: the search direction s2 = z1 + delta
+            // s1 and the inner right-hand side G s2 = G z1 + delta G s1, both
+            // from z1 and s1 (complete since the last barrier)
+            const double delta = cycle > 0 ? r1z1 / r0z0 : 0.0;
+            for (int e = gtid; e < nel; e += gsize) {
+                const double z = vldd<kCO>(gp(z1) + e);
+                vstd<kCO>(gp(s2) + e, cycle == 0 ? z : z + delta * vldd<kCO>(gp(s1) + e));
             }
-            ok = csync(U.gs, gen, ok);
-            double delta = 0.0;
-            {
-                double o[2];
-                reduce_collect<2>(U.rb, 4, gridDim.x, o, lds);
-                r1z1 = o[0];
-                if (cycle > 0) delta = r1z1 / o[1];
+            for (int i = rows.b; i < rows.e; i += rows.s) {
+                double a = g_row_ld(L, i, [&](int e) { return vldd<kCO>(gp(z1) + e); });
+                if (cycle > 0) a += delta * g_row_ld(L, i, [&](int e) { return vldd<kCO>(gp(s1) + e); });
+                U.Ahv[i] = a;
             }
-            for (int e = gtid; e < nel; e += gsize)
-                vstd<kCO>(s2 + e, cycle == 0 ? z1[e] : z1[e] + delta * vldd<kCO>(s1 + e));
-            ok = csync(U.gs, gen, ok);
-            for (int i = gtid; i < neq; i += gsize) {
-                U.Ahv[i] = g_row_ld(L, i, [&](int e) { return vldd<kCO>(s2 + e); });
-                U.u1[i] = 0.0;
-            }
-            ok = ok && grid_sync(U.gs, gen);   // the inner solve reads Ahv on its own rows
             // relaxed inner accuracy (P.relax > 0): the update this solve
             // feeds is about dv |d| in size, so its defect matters in
             // proportion; the tolerance grows as relax / dv up to relax_max,
@@ -1308,9 +1302,12 @@ __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
             if (P.relax > 0.0) rtol = fmin(P.relax_max, fmax(P.inner_rtol, P.relax / fmax(dv, 1e-300)));
         }
         MCL_PHASE(6);
-        inner += inner_mixed(P, U.Ahv, U.u1, true, U.gs, gen, ok, lds, mgl, spmv, floor2, nbr, rtol);
+        if (L.nh > 0) ok = ok && grid_sync(U.gs, gen);   // the solve's start reads b across rows
+        const int it = inner_mixed(P, U.Ahv, U.u1, U.gs, gen, ok, lds, mgl, ffc, ff, stage == 0, nbr, rtol);
+        inner += it;
+        spmv += it;
         if (stage != 1) {
-            for (int i = gtid; i < neq; i += gsize) U.d[i] += U.u1[i];
+            for (int i = gtid; i < neq; i += gsize) U.d[i] += vldd<kCO>(u1 + i);
             ok = ok && grid_sync(U.gs, gen);
             if (stage == 2) break;
             if (P.probe) {
@@ -1334,15 +1331,33 @@ __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
                 }
                 return;
             }
-            for (int e = gtid; e < nel; e += gsize) r1[e] = gt_elem(L, e, U.d);
+            // Schur residual r1 = G^T d, z1 = M_S r1 and r1.z1 for the first
+            // Uzawa iteration
+            {
+                dd acc[1] = {dd_zero()};
+                for (int e = gtid; e < nel; e += gsize) {
+                    const double r = gt_elem(L, e, U.d);
+                    const double z = U.schur ? r / U.BAB[e] : U.BAB[e] * r;
+                    r1[e] = r;
+                    vstd<kCO>(gp(z1) + e, z);
+                    acc[0] = dd_add_prod(acc[0], r, z);
+                }
+                reduce_publish<1>(U.rb, 4, acc, lds);
+            }
+            ok = csync(U.gs, gen, ok);
+            {
+                double o[1];
+                reduce_collect<1>(U.rb, 4, gridDim.x, o, lds);
+                r1z1 = o[0];
+            }
             stage = 1;
         } else {
             {
                 dd acc[1] = {dd_zero()};
                 for (int e = gtid; e < nel; e += gsize) {
-                    double a = gt_elem(L, e, U.u1);
+                    double a = gt_elem_ld(L, e, [&](int r) { return vldd<kCO>(u1 + r); });
                     U.Ahp[e] = a;
-                    acc[0] = dd_add_prod(acc[0], vldd<kCO>(s2 + e), a);
+                    acc[0] = dd_add_prod(acc[0], vldd<kCO>(gp(s2) + e), a);
                 }
                 reduce_publish<1>(U.rb, 6, acc, lds);
             }
@@ -1354,30 +1369,38 @@ __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
                 alpha = r1z1 / o[0];
             }
             {
-                dd acc[4] = {dd_zero(), dd_zero(), dd_zero(), dd_zero()};
+                // updates, their norms, and the next iteration's z = M_S r
+                // with r.z (into z0's buffer, which the rotation makes z1)
+                dd acc[5] = {dd_zero(), dd_zero(), dd_zero(), dd_zero(), dd_zero()};
                 for (int e = gtid; e < nel; e += gsize) {
-                    r2[e] = r1[e] - alpha * U.Ahp[e];
-                    const double se = vldd<kCO>(s2 + e);
+                    const double rn = r1[e] - alpha * U.Ahp[e];
+                    r2[e] = rn;
+                    const double se = vldd<kCO>(gp(s2) + e);
                     double pe = U.p[e] + alpha * se;
                     U.p[e] = pe;
                     acc[0] = dd_add_prod(acc[0], se, se);
                     acc[1] = dd_add_prod(acc[1], pe, pe);
+                    const double z = U.schur ? rn / U.BAB[e] : U.BAB[e] * rn;
+                    vstd<kCO>(gp(z0) + e, z);
+                    acc[4] = dd_add_prod(acc[4], rn, z);
                 }
                 for (int i = gtid; i < neq; i += gsize) {
-                    double u = U.u1[i];
+                    double u = vldd<kCO>(u1 + i);
                     double di = U.d[i] - alpha * u;
                     U.d[i] = di;
                     acc[2] = dd_add_prod(acc[2], u, u);
                     acc[3] = dd_add_prod(acc[3], di, di);
                 }
-                reduce_publish<4>(U.rb, 8, acc, lds);
+                reduce_publish<5>(U.rb, 8, acc, lds);
             }
             ok = csync(U.gs, gen, ok);
             {
-                double o[4];
-                reduce_collect<4>(U.rb, 8, gridDim.x, o, lds);
+                double o[5];
+                reduce_collect<5>(U.rb, 8, gridDim.x, o, lds);
                 dp = alpha * sqrt(o[0] / (1.0e-32 + o[1]));
                 dv = alpha * sqrt(o[2] / (1.0e-32 + o[3]));
+                r0z0 = r1z1;
+                r1z1 = o[4];
             }
             cycle++;
             double *sh;

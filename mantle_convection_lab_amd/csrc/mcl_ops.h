@@ -48,17 +48,24 @@ __device__ __forceinline__ double g_row(const LevelDev &L, int P, const double *
     return g_row_ld(L, P, [&](int e) { return p[e]; });
 }
 
-// (G^T x)[e], GT_prod_d (Solver.c:645-666): rows of the element ascending.
-__device__ __forceinline__ double gt_elem(const LevelDev &L, int e, const double *__restrict__ x)
+// (G^T x)[e], GT_prod_d (Solver.c:645-666): rows of the element ascending,
+// with the velocity values read through load(r).
+template <typename LD>
+__device__ __forceinline__ double gt_elem_ld(const LevelDev &L, int e, LD load)
 {
     double s = 0.0;
     int lev = L.elev[e];
     for (int t = 0; t < 8; t++) {
         int r = L.gt_row[t * L.nel + e];
         if (r < 0) break;
-        s += L.gtab[lev * 8 + L.gt_p[t * L.nel + e]] * x[r];
+        s += L.gtab[lev * 8 + L.gt_p[t * L.nel + e]] * load(r);
     }
     return s;
+}
+
+__device__ __forceinline__ double gt_elem(const LevelDev &L, int e, const double *__restrict__ x)
+{
+    return gt_elem_ld(L, e, [&](int r) { return x[r]; });
 }
 
 // ------------------------------------------------------------ persistent Uzawa
