@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 #include <string>
@@ -38,6 +39,10 @@ __global__ void k_conv3_dgrad(ConvShape s, const uint16_t *dout, const uint16_t 
                               uint16_t *din, int accum);
 __global__ void k_conv3_wgrad(ConvShape s, const uint16_t *dout, const uint16_t *relu_y, const uint16_t *in,
                               float *dW, int pix_per_split);
+__global__ void k_conv3_mm(ConvShape s, int dgrad, const uint16_t *X, const uint16_t *mask, const uint16_t *Wt,
+                           const float *bias, uint16_t *Y, int flag);
+__global__ void k_conv3_mm_n32(ConvShape s, int dgrad, const uint16_t *X, const uint16_t *mask, const uint16_t *Wt,
+                               const float *bias, uint16_t *Y, int flag);
 __global__ void k_bias_grad(int M, int Cout, int cs, int coff, const uint16_t *dout, const uint16_t *relu_y,
                             float *db);
 __global__ void k_pool_fwd(int B, int H, int W, int C, const uint16_t *in, int cs_in, int ci_off, uint16_t *out,
@@ -143,13 +148,47 @@ static int make_conv(mcl_net *n, Conv &c, int Cin, int Cout, std::mt19937_64 &rn
 
 static unsigned gblocks(size_t n) { return (unsigned)std::min<size_t>((n + 255) / 256, 8192); }
 
+// The LDS-staged kernels (k_conv3_mm: 64 pixels x 64 channels a workgroup;
+// k_conv3_mm_n32: 128 x 32, for N <= 32 output channels) apply when MT
+// consecutive pixels form a rectangle of one image (TW = min(W, MT) divides
+// both W and MT, TH = MT / TW divides H) and the summed channels and the
+// staged buffer's channel stride and offset are multiples of 8 (16-byte
+// loads).  Returns MT, or 0 for the unstaged kernels; MCL_CONV_DIRECT=1
+// keeps the unstaged kernels (for comparisons).
+static int conv_staged(const ConvShape &s, int N, int Kc, int xst, int xof)
+{
+    const char *e = getenv("MCL_CONV_DIRECT");
+    if (e && atoi(e)) return 0;
+    if (Kc % 8 || xst % 8 || xof % 8) return 0;
+    const int MT = N <= 32 ? 128 : 64;
+    const int TW = s.W < MT ? s.W : MT;
+    if (TW <= 0 || s.W % TW || MT % TW || s.H % (MT / TW)) return 0;
+    return MT;
+}
+
+// one launch of the staged kernel of tile MT (64 or 128) over M pixels, N channels
+static void conv_mm(mcl_net *n, const ConvShape &s, int MT, int M, int N, int dgrad, const uint16_t *X,
+                    const uint16_t *mask, const uint16_t *Wt, const float *bias, uint16_t *Y, int flag)
+{
+    if (MT == 128)
+        hipLaunchKernelGGL(k_conv3_mm_n32, dim3(M / 128, (N + 31) / 32), dim3(256), 0, n->stream, s, dgrad, X, mask,
+                           Wt, bias, Y, flag);
+    else
+        hipLaunchKernelGGL(k_conv3_mm, dim3(M / 64, (N + 63) / 64), dim3(256), 0, n->stream, s, dgrad, X, mask, Wt,
+                           bias, Y, flag);
+}
+
 static void conv_fwd(mcl_net *n, Conv &c, const Act &in, int ci_off, Act &out, int co_off, bool relu)
 {
     ConvShape s{n->B, in.H, in.W, c.Cin, c.Cout, in.C, ci_off, out.C, co_off};
     int M = n->B * in.H * in.W;
-    dim3 grid((M + 63) / 64, (c.Cout + 63) / 64);
-    hipLaunchKernelGGL(k_conv3_fwd, grid, dim3(256), 0, n->stream, s, (const uint16_t *)in.y,
-                       (const uint16_t *)c.Wf, (const float *)c.b, out.y, relu ? 1 : 0);
+    const int MT = conv_staged(s, c.Cout, c.Cin, in.C, ci_off);
+    if (MT)
+        conv_mm(n, s, MT, M, c.Cout, 0, (const uint16_t *)in.y, nullptr, (const uint16_t *)c.Wf, (const float *)c.b,
+                out.y, relu ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_conv3_fwd, dim3((M + 63) / 64, (c.Cout + 63) / 64), dim3(256), 0, n->stream, s,
+                           (const uint16_t *)in.y, (const uint16_t *)c.Wf, (const float *)c.b, out.y, relu ? 1 : 0);
 }
 
 // Backward of a ReLU convolution: weight and bias gradients, and the data
@@ -169,8 +208,13 @@ static void conv_bwd(mcl_net *n, Conv &c, const Act &in, int ci_off, Act &out, i
                        out.C, co_off, (const uint16_t *)out.g, mask, c.db);
     if (din) {
         dim3 gd((M + 63) / 64, (c.Cin + 63) / 64);
-        hipLaunchKernelGGL(k_conv3_dgrad, gd, dim3(256), 0, n->stream, s, (const uint16_t *)out.g, mask,
-                           (const uint16_t *)c.Wd, in.g, accum ? 1 : 0);
+        const int MT = conv_staged(s, c.Cin, c.Cout, out.C, co_off);
+        if (MT)
+            conv_mm(n, s, MT, M, c.Cin, 1, (const uint16_t *)out.g, mask, (const uint16_t *)c.Wd, nullptr, in.g,
+                    accum ? 1 : 0);
+        else
+            hipLaunchKernelGGL(k_conv3_dgrad, gd, dim3(256), 0, n->stream, s, (const uint16_t *)out.g, mask,
+                               (const uint16_t *)c.Wd, in.g, accum ? 1 : 0);
     }
 }
 

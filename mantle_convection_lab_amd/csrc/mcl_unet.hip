@@ -111,6 +111,139 @@ __global__ void __launch_bounds__(256) k_conv3_fwd(ConvShape s, const uint16_t *
         }
 }
 
+// ------------------------------------------- LDS-staged forward / data grad
+// Implicit-GEMM 3x3 convolution on v_mfma_f32_16x16x32_bf16 (gfx950) with the
+// operands staged in LDS:
+//   forward (dgrad = 0): Y[m, n] = relu?(bias[n] + sum_{tap,c} X[m + off(tap), c] Wt[n][tap][c])
+//     with X = in, Wt = Wf, Y = out;
+//   data grad (dgrad = 1): Y[m, n] (+)= sum_{tap,c} X[m - off(tap), c] Wt[n][tap][c]
+//     with X = dout (times y > 0 when mask != NULL), Wt = Wd, Y = din.
+// A workgroup (4 waves, each a 32 x 32 block of 2 x 2 MFMA tiles) computes MT
+// pixels x NT channels: 64 x 64, or 128 x 32 for layers of at most 32 output
+// channels (every wave busy).  Its MT pixels are a TH x TW rectangle of one
+// image (TW = min(W, MT); the host checks that the shape allows it).  For
+// every 32-channel chunk of the summed channels it stages the (TH + 2) x
+// (TW + 2) patch of X and the chunk's weights of its NT output channels for
+// all nine taps in LDS (rows padded by 16 bytes), then runs the nine taps, so
+// a staged value feeds nine products instead of being fetched for each.
+// Operand layout of the instruction (K = 32): A[i][k] in lane i + 16 (k / 8),
+// element k % 8; B[k][j] in lane j + 16 (k / 8), element k % 8; D[i][j] in
+// lane j + 16 (i / 4), element i % 4.
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+constexpr int CV_KC = 32;            // summed channels per stage
+constexpr int CV_ROW = CV_KC + 8;    // bf16 words per staged pixel or weight row
+
+__device__ __forceinline__ uint4 mask_bf16x8(uint4 g, uint4 y)
+{
+    unsigned *gv = (unsigned *)&g;
+    const unsigned *yv = (const unsigned *)&y;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        unsigned lo = gv[q] & 0xffffu, hi = gv[q] >> 16;
+        if (bf2f((uint16_t)(yv[q] & 0xffffu)) <= 0.f) lo = 0;
+        if (bf2f((uint16_t)(yv[q] >> 16)) <= 0.f) hi = 0;
+        gv[q] = lo | (hi << 16);
+    }
+    return g;
+}
+
+template <int MT, int NT>
+__device__ __forceinline__ void conv3_mm(const ConvShape &s, int dgrad, const uint16_t *__restrict__ X,
+                                         const uint16_t *__restrict__ mask, const uint16_t *__restrict__ Wt,
+                                         const float *__restrict__ bias, uint16_t *__restrict__ Y, int flag)
+{
+    static_assert((MT / 32) * (NT / 32) == 4, "four waves of 32 x 32");
+    constexpr int NW = NT / 32;
+    __shared__ __attribute__((aligned(16))) uint16_t xs[3 * (MT + 2) * CV_ROW];
+    __shared__ __attribute__((aligned(16))) uint16_t ws[9 * NT * CV_ROW];
+    const int Kc = dgrad ? s.Cout : s.Cin, N = dgrad ? s.Cin : s.Cout;
+    const int xst = dgrad ? s.cs_out : s.cs_in, xof = dgrad ? s.co_off : s.ci_off;
+    const int yst = dgrad ? s.cs_in : s.cs_out, yof = dgrad ? s.ci_off : s.co_off;
+    const int sg = dgrad ? -1 : 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave / NW, wn = wave % NW;
+    const int TW = s.W < MT ? s.W : MT, TH = MT / TW, PW = TW + 2, np = (TH + 2) * PW;
+    const int m0 = blockIdx.x * MT, n0 = blockIdx.y * NT;
+    const int img = m0 / (s.H * s.W), r0 = m0 - img * s.H * s.W;
+    const int h0 = r0 / s.W, w0 = r0 - h0 * s.W;
+    const bool active = n0 + wn * 32 < N;
+    f4 acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) acc[a][b] = f4{0, 0, 0, 0};
+    // this lane's pixels (patch coordinates of the tap-free position)
+    int py[2], px[2];
+    for (int a = 0; a < 2; a++) {
+        const int ml = wm * 32 + a * 16 + (lane & 15);
+        py[a] = ml / TW + 1;
+        px[a] = ml - (py[a] - 1) * TW + 1;
+    }
+    const int kq = 8 * (lane >> 4);
+    for (int c0 = 0; c0 < Kc; c0 += CV_KC) {
+        __syncthreads();   // the previous chunk's products are done with the LDS
+        for (int t = threadIdx.x; t < np * 4; t += 256) {
+            const int p = t >> 2, q = t & 3, c = c0 + 8 * q;
+            const int hh = h0 - 1 + p / PW, ww = w0 - 1 + p % PW;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (c < Kc && hh >= 0 && hh < s.H && ww >= 0 && ww < s.W) {
+                const size_t idx = ((size_t)(img * s.H + hh) * s.W + ww) * xst + xof + c;
+                v = *(const uint4 *)(X + idx);
+                if (mask) v = mask_bf16x8(v, *(const uint4 *)(mask + idx));
+            }
+            *(uint4 *)(xs + p * CV_ROW + 8 * q) = v;
+        }
+        for (int t = threadIdx.x; t < 9 * NT * 4; t += 256) {
+            const int q = t & 3, row = t >> 2, nl = row % NT, tap = row / NT, c = c0 + 8 * q;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (c < Kc && n0 + nl < N) v = *(const uint4 *)(Wt + ((size_t)(n0 + nl) * 9 + tap) * Kc + c);
+            *(uint4 *)(ws + (tap * NT + nl) * CV_ROW + 8 * q) = v;
+        }
+        __syncthreads();
+        if (active) {
+            for (int tap = 0; tap < 9; tap++) {
+                const int dh = sg * (tap / 3 - 1), dw = sg * (tap % 3 - 1);
+                bf8 A[2], B[2];
+                for (int a = 0; a < 2; a++)
+                    A[a] = *(const bf8 *)(xs + ((py[a] + dh) * PW + px[a] + dw) * CV_ROW + kq);
+                for (int b = 0; b < 2; b++)
+                    B[b] = *(const bf8 *)(ws + (tap * NT + wn * 32 + b * 16 + (lane & 15)) * CV_ROW + kq);
+                for (int a = 0; a < 2; a++)
+                    for (int b = 0; b < 2; b++)
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[a], B[b], acc[a][b], 0, 0, 0);
+            }
+        }
+    }
+    if (!active) return;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) {
+            const int n = n0 + wn * 32 + b * 16 + (lane & 15);
+            if (n >= N) continue;
+            const float bi = (!dgrad && bias) ? bias[n] : 0.f;
+            for (int e = 0; e < 4; e++) {
+                const size_t idx = (size_t)(m0 + wm * 32 + a * 16 + (lane >> 4) * 4 + e) * yst + yof + n;
+                float v = acc[a][b][e] + bi;
+                if (!dgrad && flag) v = fmaxf(v, 0.f);
+                if (dgrad && flag) v += bf2f(Y[idx]);
+                Y[idx] = f2bf(v);
+            }
+        }
+}
+
+__global__ void __launch_bounds__(256) k_conv3_mm(ConvShape s, int dgrad, const uint16_t *__restrict__ X,
+                                                  const uint16_t *__restrict__ mask, const uint16_t *__restrict__ Wt,
+                                                  const float *__restrict__ bias, uint16_t *__restrict__ Y, int flag)
+{
+    conv3_mm<64, 64>(s, dgrad, X, mask, Wt, bias, Y, flag);
+}
+
+__global__ void __launch_bounds__(256) k_conv3_mm_n32(ConvShape s, int dgrad, const uint16_t *__restrict__ X,
+                                                      const uint16_t *__restrict__ mask,
+                                                      const uint16_t *__restrict__ Wt, const float *__restrict__ bias,
+                                                      uint16_t *__restrict__ Y, int flag)
+{
+    conv3_mm<128, 32>(s, dgrad, X, mask, Wt, bias, Y, flag);
+}
+
 // ---------------------------------------------------------------- data grad
 // din[m, ci] = sum_{tap,co} g[m - off(tap), co] Wd[ci][tap][co], g = dout * (y > 0)
 // when relu_y != NULL.  Output channels ci of the input buffer (stride/offset of

@@ -55,6 +55,14 @@ class Net:
         assert p.size == self.n
         self._ck(self.L.mcl_net_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))))
 
+    def forward(self, x):
+        """Network output (B, H, W, 3) for inputs x (B, H, W, 2), through host buffers."""
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros(x.shape[:3] + (3,), np.float32)
+        self._ck(self.L.mcl_net_forward(self.h, x.ctypes.data_as(C.POINTER(C.c_float)),
+                                        y.ctypes.data_as(C.POINTER(C.c_float))))
+        return y
+
     def capture(self, ctx):
         rc = self.L.mcl_net_capture(ctx.h, self.h)
         if rc:
@@ -213,7 +221,22 @@ def main(argv=None):
     ap.add_argument("--eval-steps", type=int, default=3)
     ap.add_argument("--eval-only", default=None, help="evaluate these weights on held-out runs; no training")
     ap.add_argument("--eval-out", default=None, help="write the evaluation (JSON) to this file")
+    ap.add_argument("--time-forward", type=int, default=0, metavar="REPS",
+                    help="run REPS forward passes of an untrained network (side x side, --batch) and exit")
     args = ap.parse_args(argv)
+    if args.time_forward:
+        net = Net(args.side, args.side, args.batch, seed=args.seed)
+        x = np.random.default_rng(args.seed).uniform(0, 1, (args.batch, args.side, args.side, 2))
+        for _ in range(2):
+            net.forward(x)
+        t0 = time.perf_counter()
+        for _ in range(args.time_forward):
+            net.forward(x)
+        ms = (time.perf_counter() - t0) / args.time_forward * 1e3
+        net.close()
+        print(json.dumps({"side": args.side, "batch": args.batch, "reps": args.time_forward,
+                          "ms_per_forward_with_host_copies": ms}), flush=True)
+        return 0
     if args.eval_only:
         ev = evaluate(args.eval_only, args.side, args.eval_runs, args.eval_steps, seed=args.seed + 1000)
         summ = summarize(ev)

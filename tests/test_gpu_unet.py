@@ -130,3 +130,65 @@ def test_unet_forward_backward_adam_vs_torch():
         assert np.max(np.abs(p2[big] - expect[big])) < 1e-3 * lr + 1e-7
     finally:
         L.mcl_net_destroy(net)
+
+
+def test_staged_convolutions_match_direct_and_torch():
+    """At 64 x 64 every 3x3 convolution runs the LDS-staged K = 32 MFMA kernel
+    (64-pixel tiles of 1 x 64, 2 x 32, 4 x 16 and 8 x 8 pixels); the forward
+    output and the gradients of one training step match the unstaged kernels
+    (MCL_CONV_DIRECT=1) to bf16 rounding and PyTorch's FP64 result to the
+    bars of the test above."""
+    import os
+    import torch
+    L = _lib()
+    H = W = 64
+    B = 1
+    rng = np.random.default_rng(11)
+    x = np.zeros((B, H, W, 2), np.float32)
+    x[..., 0] = rng.uniform(0, 1, (B, H, W))
+    x[..., 1] = rng.uniform(-3, 0, (B, H, W))
+    t = rng.normal(0, 1, (B, H, W, 3)).astype(np.float32)
+    out = {}
+    for mode in ("staged", "direct"):
+        net = C.c_void_p()
+        assert L.mcl_net_create(H, W, B, 7, C.byref(net)) == 0
+        try:
+            if mode == "direct":
+                os.environ["MCL_CONV_DIRECT"] = "1"
+            print(f"[gpu-unet] {mode} forward and gradients at 64 x 64", flush=True)
+            n = L.mcl_net_num_params(net)
+            p = np.zeros(n, np.float32)
+            assert L.mcl_net_get_params(net, _fp(p)) == 0
+            y = np.zeros((B, H, W, 3), np.float32)
+            assert L.mcl_net_forward(net, _fp(x), _fp(y)) == 0
+            loss = C.c_float()
+            assert L.mcl_net_train_step(net, _fp(x), _fp(t), 0.0, C.byref(loss)) == 0
+            g = np.zeros(n, np.float32)
+            assert L.mcl_net_get_grads(net, _fp(g)) == 0
+            out[mode] = (p, y, g)
+        finally:
+            os.environ.pop("MCL_CONV_DIRECT", None)
+            L.mcl_net_destroy(net)
+    p, ys, gs = out["staged"]
+    _, yd, gd = out["direct"]
+    assert np.array_equal(p, out["direct"][0])
+    d = np.linalg.norm(ys - yd) / np.linalg.norm(yd)
+    print(f"  staged vs direct forward: rel diff {d:.2e}", flush=True)
+    assert d < 2e-2
+    fwd, ws, wh, bh = _torch_net(p.astype(np.float64))
+    xt = torch.zeros((B, 16, H, W), dtype=torch.float64)
+    xt[:, :2] = torch.tensor(x, dtype=torch.float64).permute(0, 3, 1, 2)
+    yt = fwd(xt)
+    yref = yt.detach().permute(0, 2, 3, 1).numpy()
+    err = np.linalg.norm(ys - yref) / np.linalg.norm(yref)
+    print(f"  staged forward vs FP64: rel err {err:.2e}", flush=True)
+    assert err < 3e-2
+    lt = ((yt - torch.tensor(t, dtype=torch.float64).permute(0, 3, 1, 2)) ** 2).mean()
+    lt.backward()
+    off = 0
+    for i, r in enumerate(_flat_grads(ws, wh, bh)):
+        es = np.linalg.norm(gs[off: off + r.size] - r) / max(np.linalg.norm(r), 1e-30)
+        ed = np.linalg.norm(gd[off: off + r.size] - r) / max(np.linalg.norm(r), 1e-30)
+        off += r.size
+        print(f"  grad block {i}: staged {es:.2e} direct {ed:.2e}", flush=True)
+        assert es < 1e-1, i
