@@ -538,6 +538,99 @@ __device__ __forceinline__ Own own_block(int n)
     return Own{R, b0, max(0, min(R, n - b0))};
 }
 
+// ------------------------------------ one-row-per-thread two-sweep phases
+// On the coarse grid levels every list of a workgroup fits its MCL_BLOCK
+// threads, so a thread holds at most one row of each.  These forms issue all
+// loads of a phase that do not wait on another workgroup at its start -- the
+// K rows' local columns and values, 1/diag, right-hand sides, transfer maps --
+// right after the row indices, so the phase costs three dependent memory
+// round trips instead of two per list; a K row needed by two sweeps is loaded
+// once.  Same operations in the same order as the general forms below.
+
+// a K row of width 18 in registers: its columns as local indices and values
+struct KRow18 {
+    int c[18];
+    float v[18];
+};
+
+__device__ __forceinline__ void krow_load(KRow18 &k, const MCL_G unsigned short *map, int ms, int mi,
+                                          const GLevel &F, int row)
+{
+#pragma unroll
+    for (int u = 0; u < 18; u++) {
+        k.c[u] = map[u * ms + mi];
+        k.v[u] = F.val[u * F.n + row];
+    }
+}
+
+template <typename XP>
+__device__ __forceinline__ float krow_dot(const KRow18 &k, XP x)
+{
+    float s = 0.f;
+#pragma unroll
+    for (int u = 0; u < 18; u++) s += k.v[u] * x[k.c[u]];
+    return s;
+}
+
+// deep_down for lists of at most MCL_BLOCK rows (false: not applicable)
+template <typename LB>
+__device__ __forceinline__ bool deep_down1(const GLevel &F, const GLevel &C, LB buf, float wf, float wc)
+{
+    const Own o = own_block(F.n), oc = own_block(C.n);
+    const int g = blockIdx.x;
+    const int t0 = F.th_ptr[g], nth = F.th_ptr[g + 1] - t0;
+    const int u0 = F.uh_ptr[g], nuh = F.uh_ptr[g + 1] - u0;
+    const int f0 = F.hd_ptr[g], nf = F.hd_ptr[g + 1] - f0;
+    const int nT = o.R + nth, nU = nT + nuh;
+    if (F.WK != 18 || C.rw > 9 || nU > MCL_BLOCK || nf > MCL_BLOCK || oc.R > MCL_BLOCK) return false;
+    const int t = threadIdx.x;
+    const auto xU = buf;
+    const auto xT = buf + nU;
+    const auto rF = xT + nT;
+    // this thread's row of U (T is its prefix) and of F
+    const bool inU = t < nU && (t >= o.R || t < o.no), inT = inU && t < nT, inF = t < nf;
+    const int ru = inU ? (t < o.R ? o.b0 + t : t < nT ? F.th_row[t0 + t - o.R] : F.uh_row[u0 + t - nT]) : 0;
+    const int rf = inF ? F.hd_row[f0 + t] : 0;
+    // every load that does not wait on another workgroup
+    KRow18 kt, kf;
+    if (inT) krow_load(kt, F.mt + (size_t)((long long)g * o.R + t0) * F.WK, nT, t, F, ru);
+    if (inF) krow_load(kf, F.mf + (size_t)f0 * F.WK, nf, t, F, rf);
+    const float dT = inT ? F.dinv[ru] : 0.f;
+    const float bT = inT ? vld<kCO>(F.b + ru) : 0.f;
+    const float bF = inF ? vld<kCO>(F.b + rf) : 0.f;
+    const float xu = inU ? vld<kCO>(sweep_buf(F, 1) + ru) : 0.f;
+    const int c = oc.b0 + t;
+    const bool inC = t < oc.no;
+    int l[9];
+    float rv[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const bool in = inC && k < C.rw;
+        l[k] = in ? C.rl[k * C.n + c] : -1;
+        rv[k] = in ? C.rval[k * C.n + c] : 0.f;
+    }
+    const float dC = inC ? C.dinv[c] : 0.f;
+    if (inU) xU[t] = xu;
+    __syncthreads();
+    if (inT) {
+        const float v = xU[t] + wf * dT * (bT - krow_dot(kt, xU));
+        xT[t] = v;
+        if (t < o.R) vst<kCO>(sweep_buf(F, 2) + ru, v);
+    }
+    __syncthreads();
+    if (inF) rF[t] = bF - krow_dot(kf, xT);
+    __syncthreads();
+    if (inC) {
+        float sc = 0.f;
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+            if (l[k] >= 0) sc += rv[k] * rF[l[k]];
+        vst<kCO>(C.b + c, sc);
+        vst<kCO>(C.x + c, wc * dC * sc);
+    }
+    return true;
+}
+
 // Two-sweep down-leg phase of grid level F (pre = 2) above a coarse grid
 // level C: x_1 on U into LDS, sweep 2 on T (stored for the workgroup's own
 // rows), the residual on F, then b_c = R r and the coarse first sweep.  The
@@ -545,6 +638,7 @@ __device__ __forceinline__ Own own_block(int n)
 template <typename LB>
 __device__ __forceinline__ void deep_down(const GLevel &F, const GLevel &C, LB buf, float wf, float wc)
 {
+    if (deep_down1(F, C, buf, wf, wc)) return;
     const Own o = own_block(F.n);
     const int g = blockIdx.x;
     const int t0 = F.th_ptr[g], nth = F.th_ptr[g + 1] - t0;
@@ -611,6 +705,48 @@ __device__ __forceinline__ float kw_row(const GLevel &F, const Halo2 &h, int t, 
     return ell32_map(F, F.hm + (size_t)h.h10 * F.WK, h.nh1, t - h.o.R, row, xV);
 }
 
+// The thread's row of V (-1: none), and its K row when it is a row of W:
+// O rows' columns through kl, H1 rows' through hm (both index V, whose
+// prefix W is; an O row's kl columns also index W alone).
+__device__ __forceinline__ int v_row1(const GLevel &F, const Halo2 &h, int t, KRow18 &kw, float &dW, float &bW)
+{
+    const bool inV = t < h.nV && (t >= h.o.R || t < h.o.no);
+    const int rv = inV ? vrow(F, h, t) : -1;
+    dW = 0.f;
+    bW = 0.f;
+    if (inV && t < h.nW) {
+        if (t < h.o.R) krow_load(kw, F.kl, F.n, rv, F, rv);
+        else krow_load(kw, F.hm + (size_t)h.h10 * F.WK, h.nh1, t - h.o.R, F, rv);
+        dW = F.dinv[rv];
+        bW = vld<kCO>(F.b + rv);
+    }
+    return rv;
+}
+
+// deep_down_last for lists of at most MCL_BLOCK rows (false: not applicable)
+template <typename LB>
+__device__ __forceinline__ bool deep_down_last1(const GLevel &F, const Halo2 &h, LB buf, float wf)
+{
+    if (F.WK != 18 || h.nV > MCL_BLOCK) return false;
+    const int t = threadIdx.x;
+    const auto xV = buf;
+    const auto xW = buf + h.nV;
+    KRow18 kw;
+    float dW, bW;
+    const int rv = v_row1(F, h, t, kw, dW, bW);
+    const float xv = rv >= 0 ? vld<kCO>(sweep_buf(F, 1) + rv) : 0.f;
+    if (rv >= 0) xV[t] = xv;
+    __syncthreads();
+    if (rv >= 0 && t < h.nW) {
+        const float v = xV[t] + wf * dW * (bW - krow_dot(kw, xV));
+        xW[t] = v;
+        if (t < h.o.R) vst<kCO>(sweep_buf(F, 2) + rv, v);
+    }
+    __syncthreads();
+    if (t < h.o.no) vst<kCO>(F.r + rv, bW - krow_dot(kw, xW));
+    return true;
+}
+
 // Two-sweep down-leg phase of the grid level just above the LDS levels
 // (pre = 2): x_1 on V, sweep 2 on W (stored for the workgroup's own rows),
 // the residual on its own rows (F.r, which every workgroup then restricts
@@ -619,6 +755,7 @@ template <typename LB>
 __device__ __forceinline__ void deep_down_last(const GLevel &F, LB buf, float wf)
 {
     const Halo2 h = halo2(F);
+    if (deep_down_last1(F, h, buf, wf)) return;
     const auto xV = buf;
     const auto xW = buf + h.nV;
     const auto x1 = sweep_buf(F, 1);
@@ -645,11 +782,62 @@ __device__ __forceinline__ void deep_down_last(const GLevel &F, LB buf, float wf
 // z, zeroed on hanging equations, with sum r_i z_i published to `slot`.  The
 // operations and their order are those of prolong_add, jacobi and
 // last_sweep_rz.
+// deep_up for lists of at most MCL_BLOCK rows (false: not applicable)
+template <bool CC, typename XC, typename LB>
+__device__ __forceinline__ bool deep_up1(const GLevel &F, const Halo2 &h, XC xc, LB buf, float w, bool finest,
+                                         const MixedParams &P, const double *__restrict__ r, int slot,
+                                         dd (*lds)[MCL_BLOCK / MCL_WAVE])
+{
+    if (F.WK != 18 || h.nV > MCL_BLOCK) return false;
+    const int t = threadIdx.x;
+    const auto xV = buf;
+    const auto xW = buf + h.nV;
+    KRow18 kw;
+    float dW, bW;
+    const int rv = v_row1(F, h, t, kw, dW, bW);
+    const int u = rv >= 0 ? rv : 0;
+    const int m = rv >= 0 ? F.pn[u] : 0;
+    int lc[4];
+    float gv[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) lc[k] = rv >= 0 ? F.pcol[k * F.n + u] : -1;
+    const float x2v = rv >= 0 ? vld<kCO>(sweep_buf(F, 2) + u) : 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; k++) gv[k] = (k < m && lc[k] >= 0) ? vld<CC>(xc + lc[k]) : 0.f;
+    if (rv >= 0) {
+        float v = x2v;
+        if (m != 0) {
+            float s = 0.f;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (k < m && lc[k] >= 0) s += gv[k];
+            v = v + s / (float)m;
+        }
+        xV[t] = v;
+    }
+    __syncthreads();
+    if (rv >= 0 && t < h.nW) xW[t] = xV[t] + w * dW * (bW - krow_dot(kw, xV));
+    __syncthreads();
+    dd acc[1] = {dd_zero()};
+    if (t < h.o.no) {
+        float z = xW[t] + w * dW * (bW - krow_dot(kw, xW));
+        if (finest) {
+            const LevelDev &L = P.U.L;
+            if (L.nh > 0 && L.hp_n[rv] != 0) z = 0.f;  // z lives in the constrained space
+            acc[0] = dd_add_prod(acc[0], r[rv], (double)z);
+        }
+        vst<kCO>(F.x + rv, z);
+    }
+    if (finest) reduce_publish<1>(P.U.rb, slot, acc, lds);
+    return true;
+}
+
 template <bool CC, typename XC, typename LB>
 __device__ __forceinline__ void deep_up(const GLevel &F, XC xc, LB buf, float w, bool finest, const MixedParams &P,
                                         const double *__restrict__ r, int slot, dd (*lds)[MCL_BLOCK / MCL_WAVE])
 {
     const Halo2 h = halo2(F);
+    if (deep_up1<CC>(F, h, xc, buf, w, finest, P, r, slot, lds)) return;
     const auto xV = buf;
     const auto xW = buf + h.nV;
     const auto x2 = sweep_buf(F, 2);
