@@ -43,6 +43,8 @@ __global__ void k_conv3_mm(ConvShape s, int dgrad, const uint16_t *X, const uint
                            const float *bias, uint16_t *Y, int flag);
 __global__ void k_conv3_mm_n32(ConvShape s, int dgrad, const uint16_t *X, const uint16_t *mask, const uint16_t *Wt,
                                const float *bias, uint16_t *Y, int flag);
+__global__ void k_conv3_wgrad_mm(ConvShape s, const uint16_t *dout, const uint16_t *relu_y, const uint16_t *in,
+                                 float *dW, float *db, int tiles_per_split);
 __global__ void k_bias_grad(int M, int Cout, int cs, int coff, const uint16_t *dout, const uint16_t *relu_y,
                             float *db);
 __global__ void k_pool_fwd(int B, int H, int W, int C, const uint16_t *in, int cs_in, int ci_off, uint16_t *out,
@@ -53,10 +55,9 @@ __global__ void k_up_fwd(int B, int H, int W, int C, const uint16_t *in, int cs_
                          int cs_out, int co_off);
 __global__ void k_up_bwd(int B, int H, int W, int C, const uint16_t *dout, int cs_out, int co_off, uint16_t *din,
                          int cs_in, int ci_off);
-__global__ void k_head_fwd(int M, int C, int O, const uint16_t *x, int cs, const float *Wh, const float *bh,
-                           float *y);
-__global__ void k_head_bwd(int M, int C, int O, const uint16_t *x, int cs, const float *Wh, const float *y,
-                           const float *tgt, float scale, uint16_t *dx, float *dWh, float *dbh, float *loss);
+__global__ void k_head_fwd(int M, const uint16_t *x, int cs, const float *Wh, const float *bh, float *y);
+__global__ void k_head_bwd(int M, const uint16_t *x, int cs, const float *Wh, const float *y, const float *tgt,
+                           float scale, uint16_t *dx, float *dWh, float *dbh, float *loss);
 __global__ void k_adam(int n, float *w, const float *g, float *m1, float *m2, float lr, float b1, float b2, float eps,
                        float c1, float c2);
 __global__ void k_weights_bf16(int Cout, int Cin, const float *w, uint16_t *Wf, uint16_t *Wd);
@@ -198,14 +199,26 @@ static void conv_bwd(mcl_net *n, Conv &c, const Act &in, int ci_off, Act &out, i
     ConvShape s{n->B, in.H, in.W, c.Cin, c.Cout, in.C, ci_off, out.C, co_off};
     int M = n->B * in.H * in.W;
     const uint16_t *mask = out.y;
-    int splits = std::max(1, std::min(256, M / 4096));
-    int pps = ((M + splits - 1) / splits + 15) / 16 * 16;
-    splits = (M + pps - 1) / pps;
-    dim3 gw((c.Cout + 63) / 64, (9 * c.Cin + 63) / 64, splits);
-    hipLaunchKernelGGL(k_conv3_wgrad, gw, dim3(256), 0, n->stream, s, (const uint16_t *)out.g, mask,
-                       (const uint16_t *)in.y, c.dw, pps);
-    hipLaunchKernelGGL(k_bias_grad, dim3(std::min(1024, (M + 63) / 64)), dim3(256), 0, n->stream, M, c.Cout,
-                       out.C, co_off, (const uint16_t *)out.g, mask, c.db);
+    const int TW = in.W < 64 ? in.W : 64;
+    if (conv_staged(s, 64, c.Cin, in.C, ci_off) == 64 && TW % 8 == 0 && out.C % 8 == 0 && co_off % 8 == 0) {
+        // staged weight gradient (and bias gradient): workgroups of 64 x 32 x
+        // 9 taps, about 512 in all, each summing at least 8 tiles of 64 pixels
+        const int tiles = M / 64, nbn = (c.Cout + 63) / 64, nbc = (c.Cin + 31) / 32;
+        int splits = std::max(1, std::min(tiles / 8, 512 / (nbn * nbc)));
+        const int tps = (tiles + splits - 1) / splits;
+        splits = (tiles + tps - 1) / tps;
+        hipLaunchKernelGGL(k_conv3_wgrad_mm, dim3(nbn, nbc, splits), dim3(256), 0, n->stream, s,
+                           (const uint16_t *)out.g, mask, (const uint16_t *)in.y, c.dw, c.db, tps);
+    } else {
+        int splits = std::max(1, std::min(256, M / 4096));
+        int pps = ((M + splits - 1) / splits + 15) / 16 * 16;
+        splits = (M + pps - 1) / pps;
+        dim3 gw((c.Cout + 63) / 64, (9 * c.Cin + 63) / 64, splits);
+        hipLaunchKernelGGL(k_conv3_wgrad, gw, dim3(256), 0, n->stream, s, (const uint16_t *)out.g, mask,
+                           (const uint16_t *)in.y, c.dw, pps);
+        hipLaunchKernelGGL(k_bias_grad, dim3(std::min(1024, (M + 63) / 64)), dim3(256), 0, n->stream, M, c.Cout,
+                           out.C, co_off, (const uint16_t *)out.g, mask, c.db);
+    }
     if (din) {
         dim3 gd((M + 63) / 64, (c.Cin + 63) / 64);
         const int MT = conv_staged(s, c.Cin, c.Cout, out.C, co_off);
@@ -353,8 +366,8 @@ static void forward(mcl_net *n)
                        (const uint16_t *)n->d2b.y, 64, 0, n->cat1.y, 96, 0);
     conv_fwd(n, n->dec1a, n->cat1, 0, n->d1a, 0, true);
     conv_fwd(n, n->dec1b, n->d1a, 0, n->d1b, 0, true);
-    hipLaunchKernelGGL(k_head_fwd, dim3(gblocks(M)), dim3(256), 0, n->stream, (int)M, 32, 3,
-                       (const uint16_t *)n->d1b.y, 32, (const float *)n->wh, (const float *)n->bh, n->y);
+    hipLaunchKernelGGL(k_head_fwd, dim3(gblocks(M)), dim3(256), 0, n->stream, (int)M, (const uint16_t *)n->d1b.y,
+                       32, (const float *)n->wh, (const float *)n->bh, n->y);
 }
 
 static void zero_grads(mcl_net *n)
@@ -374,7 +387,7 @@ static void backward(mcl_net *n)
     const int B = n->B;
     const int M = B * n->H * n->W;
     float scale = 2.0f / (3.0f * (float)M);
-    hipLaunchKernelGGL(k_head_bwd, dim3(std::min(1024, (M + 255) / 256)), dim3(256), 0, n->stream, M, 32, 3,
+    hipLaunchKernelGGL(k_head_bwd, dim3(std::min(1024, (M + 255) / 256)), dim3(256), 0, n->stream, M,
                        (const uint16_t *)n->d1b.y, 32, (const float *)n->wh, (const float *)n->y,
                        (const float *)n->tgt, scale, n->d1b.g, n->dwh, n->dbh, n->loss);
     conv_bwd(n, n->dec1b, n->d1a, 0, n->d1b, 0, true, false);

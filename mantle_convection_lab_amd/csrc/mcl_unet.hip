@@ -394,6 +394,115 @@ __global__ void __launch_bounds__(256) k_conv3_wgrad(ConvShape s, const uint16_t
         }
 }
 
+// LDS-staged weight gradient on v_mfma_f32_16x16x32_bf16 (K = pixels):
+// dW[n][tap][c] += sum_m G[m, n] X[m + off(tap), c], G = dout (y > 0 when
+// relu_y != NULL), X = in.  A workgroup owns 64 output channels n, 32 input
+// channels c and all nine taps (waves: 16 n each, 18 tiles of 16 x 16), and
+// sums over tiles_per_split 64-pixel tiles (TH x TW rectangles of one image,
+// as k_conv3_mm); the partial sums are added to dW with FP32 atomics, and the
+// workgroups of the first channel block add the bias gradient db[n] =
+// sum_m G[m, n] (the sums of their G fragments) when db != NULL.  Per tile it
+// stages G transposed (channel-major, 64 pixels a row) and three
+// copies of the X patch shifted by -1, 0, +1 columns, channel-major, so that
+// every operand fragment -- 8 consecutive pixels of one channel -- is one
+// 16-byte LDS read.  Requires TW % 8 == 0 (the host checks the shapes).
+constexpr int WG_XP = 72;   // bf16 words per staged G row (64 pixels + pad)
+
+__global__ void __launch_bounds__(256) k_conv3_wgrad_mm(ConvShape s, const uint16_t *__restrict__ dout,
+                                                        const uint16_t *__restrict__ relu_y,
+                                                        const uint16_t *__restrict__ in, float *__restrict__ dW,
+                                                        float *__restrict__ db, int tiles_per_split)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t gt[64 * WG_XP];
+    __shared__ __attribute__((aligned(16))) uint16_t xs[3 * 32 * 3 * 72];   // largest: TW = 64, TH + 2 = 3
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int TW = s.W < 64 ? s.W : 64, TH = 64 / TW, PR = TH + 2, XP = TW + 8;
+    const int n0 = blockIdx.x * 64, c0 = blockIdx.y * 32;
+    const int ntiles = s.B * s.H * s.W / 64;
+    const int tb = blockIdx.z * tiles_per_split, te = min(ntiles, tb + tiles_per_split);
+    const bool active = n0 + 16 * wave < s.Cout;
+    const bool bias = db != nullptr && blockIdx.y == 0;
+    float bsum = 0.f;   // this lane's share of sum_m G[m, n] for n = n0 + 16 wave + lane % 16
+    f4 acc[9][2];
+    for (int t = 0; t < 9; t++)
+        for (int b = 0; b < 2; b++) acc[t][b] = f4{0, 0, 0, 0};
+    const int kq = 8 * (lane >> 4);
+    for (int tile = tb; tile < te; tile++) {
+        const int m0 = tile * 64;
+        const int img = m0 / (s.H * s.W), r0 = m0 - img * s.H * s.W;
+        const int h0 = r0 / s.W, w0 = r0 - h0 * s.W;
+        __syncthreads();   // the previous tile's products are done with the LDS
+        // G^T: 64 pixels x 64 channels (8 channels per 16-byte load)
+        for (int t = threadIdx.x; t < 64 * 8; t += 256) {
+            const int p = t >> 3, q = t & 7, n = n0 + 8 * q;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (n < s.Cout) {
+                const size_t idx = (size_t)(m0 + p) * s.cs_out + s.co_off + n;
+                v = *(const uint4 *)(dout + idx);
+                if (relu_y) v = mask_bf16x8(v, *(const uint4 *)(relu_y + idx));
+            }
+            const uint16_t *e = (const uint16_t *)&v;
+#pragma unroll
+            for (int k = 0; k < 8; k++) gt[(8 * q + k) * WG_XP + p] = e[k];
+        }
+        // X patch (PR rows x TW + 2 columns) into the three shifted copies
+        const int pw = TW + 2;
+        for (int t = threadIdx.x; t < PR * pw * 4; t += 256) {
+            const int pp = t >> 2, q = t & 3, c = c0 + 8 * q;
+            const int prow = pp / pw, pcol = pp - prow * pw;
+            const int hh = h0 - 1 + prow, ww = w0 - 1 + pcol;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (c < s.Cin && hh >= 0 && hh < s.H && ww >= 0 && ww < s.W)
+                v = *(const uint4 *)(in + ((size_t)(img * s.H + hh) * s.W + ww) * s.cs_in + s.ci_off + c);
+            const uint16_t *e = (const uint16_t *)&v;
+            // copy sh holds X at column col + sh - 1 of the tile: col = pcol - sh
+#pragma unroll
+            for (int sh = 0; sh < 3; sh++) {
+                const int col = pcol - sh;
+                if (col < 0 || col >= TW) continue;
+#pragma unroll
+                for (int k = 0; k < 8; k++) xs[((sh * 32 + 8 * q + k) * PR + prow) * XP + col] = e[k];
+            }
+        }
+        __syncthreads();
+        if (!active) continue;
+        for (int kk = 0; kk < 64; kk += 32) {
+            const bf8 A = *(const bf8 *)(gt + (16 * wave + (lane & 15)) * WG_XP + kk + kq);
+            if (bias) {
+                const uint4 a4 = *(const uint4 *)(gt + (16 * wave + (lane & 15)) * WG_XP + kk + kq);
+                const uint16_t *e = (const uint16_t *)&a4;
+#pragma unroll
+                for (int k = 0; k < 8; k++) bsum += bf2f(e[k]);
+            }
+            const int p = kk + kq, py = p / TW, px = p - py * TW;
+            for (int tap = 0; tap < 9; tap++) {
+                const int dh = tap / 3 - 1, sh = tap % 3;
+                for (int b = 0; b < 2; b++) {
+                    const bf8 B = *(const bf8 *)(xs + ((sh * 32 + 16 * b + (lane & 15)) * PR + py + dh + 1) * XP + px);
+                    acc[tap][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[tap][b], 0, 0, 0);
+                }
+            }
+        }
+    }
+    if (!active) return;
+    if (bias) {
+        bsum += __shfl_xor(bsum, 16, 64);
+        bsum += __shfl_xor(bsum, 32, 64);
+        const int n = n0 + 16 * wave + lane;
+        if (lane < 16 && n < s.Cout) atomicAdd(&db[n], bsum);
+    }
+    const int NC = 9 * s.Cin;
+    for (int tap = 0; tap < 9; tap++)
+        for (int b = 0; b < 2; b++) {
+            const int c = c0 + 16 * b + (lane & 15);
+            if (c >= s.Cin) continue;
+            for (int e = 0; e < 4; e++) {
+                const int n = n0 + 16 * wave + (lane >> 4) * 4 + e;
+                if (n < s.Cout) atomicAdd(&dW[(size_t)n * NC + tap * s.Cin + c], acc[tap][b][e]);
+            }
+        }
+}
+
 // db[co] += sum_m g[m, co]
 __global__ void k_bias_grad(int M, int Cout, int cs, int coff, const uint16_t *__restrict__ dout,
                             const uint16_t *__restrict__ relu_y, float *__restrict__ db)
@@ -448,11 +557,24 @@ __global__ void k_pool_bwd(int B, int H, int W, int C, const uint16_t *__restric
     }
 }
 
-// nearest 2x upsampling: out (B, 2H, 2W) from in (B, H, W)
+// nearest 2x upsampling: out (B, 2H, 2W) from in (B, H, W); eight channels a
+// thread (16-byte accesses) when C, the strides and the offsets allow it
 __global__ void k_up_fwd(int B, int H, int W, int C, const uint16_t *__restrict__ in, int cs_in, int ci_off,
                          uint16_t *__restrict__ out, int cs_out, int co_off)
 {
     const int Ho = 2 * H, Wo = 2 * W;
+    if (C % 8 == 0 && cs_in % 8 == 0 && ci_off % 8 == 0 && cs_out % 8 == 0 && co_off % 8 == 0) {
+        const int C8 = C / 8;
+        size_t total8 = (size_t)B * Ho * Wo * C8;
+        for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total8; t += (size_t)gridDim.x * blockDim.x) {
+            int c = (int)(t % C8) * 8;
+            size_t p = t / C8;
+            int w = p % Wo, h = (p / Wo) % Ho, n = p / ((size_t)Wo * Ho);
+            *(uint4 *)(out + p * cs_out + co_off + c) =
+                *(const uint4 *)(in + ((size_t)((n * H + h / 2) * W + w / 2)) * cs_in + ci_off + c);
+        }
+        return;
+    }
     size_t total = (size_t)B * Ho * Wo * C;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
         int c = t % C;
@@ -480,54 +602,103 @@ __global__ void k_up_bwd(int B, int H, int W, int C, const uint16_t *__restrict_
 }
 
 // ------------------------------------------------------------- head and loss
-// y[m, o] = bh[o] + sum_c Wh[o][c] x[m, c]   (1x1 convolution, FP32 output)
-__global__ void k_head_fwd(int M, int C, int O, const uint16_t *__restrict__ x, int cs, const float *__restrict__ Wh,
-                           const float *__restrict__ bh, float *__restrict__ y)
+// The 1x1 head: 32 channels -> 3 outputs (FP32).  A pixel's 32 bf16 channels
+// are four 16-byte loads (channel stride a multiple of 8).
+constexpr int HC = 32, HO = 3;
+
+__device__ __forceinline__ void load32(const uint16_t *p, float (&x)[HC])
 {
-    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x)
-        for (int o = 0; o < O; o++) {
-            float s = bh[o];
-            for (int c = 0; c < C; c++) s += Wh[o * C + c] * bf2f(x[(size_t)m * cs + c]);
-            y[(size_t)m * O + o] = s;
-        }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint4 v = *(const uint4 *)(p + 8 * q);
+        const uint16_t *e = (const uint16_t *)&v;
+#pragma unroll
+        for (int k = 0; k < 8; k++) x[8 * q + k] = bf2f(e[k]);
+    }
 }
 
-// dy = 2 (y - t) / (M O), loss partials; dx[m,c] = sum_o Wh[o][c] dy; dWh, dbh (atomics)
-__global__ void k_head_bwd(int M, int C, int O, const uint16_t *__restrict__ x, int cs, const float *__restrict__ Wh,
-                           const float *__restrict__ y, const float *__restrict__ tgt, float scale,
-                           uint16_t *__restrict__ dx, float *__restrict__ dWh, float *__restrict__ dbh,
-                           float *__restrict__ loss)
+// y[m, o] = bh[o] + sum_c Wh[o][c] x[m, c]
+__global__ void __launch_bounds__(256) k_head_fwd(int M, const uint16_t *__restrict__ x, int cs,
+                                                  const float *__restrict__ Wh, const float *__restrict__ bh,
+                                                  float *__restrict__ y)
 {
-    __shared__ float sW[8 * 64], sb[8];
-    for (int t = threadIdx.x; t < O * C + O; t += blockDim.x) {
-        if (t < O * C) sW[t] = 0.f;
-        else sb[t - O * C] = 0.f;
-    }
+    __shared__ float w[HO * HC + HO];
+    for (int t = threadIdx.x; t < HO * HC + HO; t += blockDim.x) w[t] = t < HO * HC ? Wh[t] : bh[t - HO * HC];
     __syncthreads();
-    float ls = 0.f;
     for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
-        float dy[8];
-        for (int o = 0; o < O; o++) {
-            float r = y[(size_t)m * O + o] - tgt[(size_t)m * O + o];
+        float xv[HC];
+        load32(x + (size_t)m * cs, xv);
+#pragma unroll
+        for (int o = 0; o < HO; o++) {
+            float s = w[HO * HC + o];
+#pragma unroll
+            for (int c = 0; c < HC; c++) s += w[o * HC + c] * xv[c];
+            y[(size_t)m * HO + o] = s;
+        }
+    }
+}
+
+// dy = 2 (y - t) / (M O), loss partials; dx[m,c] = sum_o Wh[o][c] dy; dWh,
+// dbh and the loss summed per thread in registers, then per workgroup, then
+// one atomic per value and workgroup.
+__global__ void __launch_bounds__(256) k_head_bwd(int M, const uint16_t *__restrict__ x, int cs,
+                                                  const float *__restrict__ Wh, const float *__restrict__ y,
+                                                  const float *__restrict__ tgt, float scale,
+                                                  uint16_t *__restrict__ dx, float *__restrict__ dWh,
+                                                  float *__restrict__ dbh, float *__restrict__ loss)
+{
+    constexpr int NV = HO * HC + HO + 1;   // dWh, dbh, loss
+    __shared__ float w[HO * HC];
+    __shared__ float red[4][NV];
+    for (int t = threadIdx.x; t < HO * HC; t += blockDim.x) w[t] = Wh[t];
+    __syncthreads();
+    float aw[HO * HC], ab[HO], ls = 0.f;
+#pragma unroll
+    for (int k = 0; k < HO * HC; k++) aw[k] = 0.f;
+#pragma unroll
+    for (int o = 0; o < HO; o++) ab[o] = 0.f;
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        float dy[HO];
+#pragma unroll
+        for (int o = 0; o < HO; o++) {
+            const float r = y[(size_t)m * HO + o] - tgt[(size_t)m * HO + o];
             ls += r * r;
             dy[o] = scale * r;
-            atomicAdd(&sb[o], dy[o]);
+            ab[o] += dy[o];
         }
-        for (int c = 0; c < C; c++) {
-            float xv = bf2f(x[(size_t)m * cs + c]);
-            float g = 0.f;
-            for (int o = 0; o < O; o++) {
-                g += Wh[o * C + c] * dy[o];
-                atomicAdd(&sW[o * C + c], dy[o] * xv);
+        float xv[HC];
+        load32(x + (size_t)m * cs, xv);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            uint4 v;
+            uint16_t *e = (uint16_t *)&v;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int c = 8 * q + k;
+                float g = 0.f;
+#pragma unroll
+                for (int o = 0; o < HO; o++) {
+                    g += w[o * HC + c] * dy[o];
+                    aw[o * HC + c] += dy[o] * xv[c];
+                }
+                e[k] = f2bf(g);
             }
-            dx[(size_t)m * cs + c] = f2bf(g);
+            *(uint4 *)(dx + (size_t)m * cs + 8 * q) = v;
         }
     }
-    atomicAdd(loss, ls);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        float v = k < HO * HC ? aw[k] : k < HO * HC + HO ? ab[k - HO * HC] : ls;
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) red[wave][k] = v;
+    }
     __syncthreads();
-    for (int t = threadIdx.x; t < O * C + O; t += blockDim.x) {
-        if (t < O * C) atomicAdd(&dWh[t], sW[t]);
-        else atomicAdd(&dbh[t - O * C], sb[t - O * C]);
+    for (int k = threadIdx.x; k < NV; k += blockDim.x) {
+        const float v = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+        if (k < HO * HC) atomicAdd(&dWh[k], v);
+        else if (k < HO * HC + HO) atomicAdd(&dbh[k - HO * HC], v);
+        else atomicAdd(loss, v);
     }
 }
 

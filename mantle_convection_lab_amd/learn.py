@@ -63,6 +63,15 @@ class Net:
                                         y.ctypes.data_as(C.POINTER(C.c_float))))
         return y
 
+    def train_step(self, x, t, lr):
+        """One Adam step on the batch (x (B, H, W, 2), targets t (B, H, W, 3)); returns the loss."""
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        loss = C.c_float()
+        self._ck(self.L.mcl_net_train_step(self.h, x.ctypes.data_as(C.POINTER(C.c_float)),
+                                           t.ctypes.data_as(C.POINTER(C.c_float)), lr, C.byref(loss)))
+        return loss.value
+
     def capture(self, ctx):
         rc = self.L.mcl_net_capture(ctx.h, self.h)
         if rc:
@@ -223,19 +232,26 @@ def main(argv=None):
     ap.add_argument("--eval-out", default=None, help="write the evaluation (JSON) to this file")
     ap.add_argument("--time-forward", type=int, default=0, metavar="REPS",
                     help="run REPS forward passes of an untrained network (side x side, --batch) and exit")
+    ap.add_argument("--time-train", type=int, default=0, metavar="REPS",
+                    help="run REPS training steps (lr 0) of an untrained network on random data and exit")
     args = ap.parse_args(argv)
-    if args.time_forward:
+    if args.time_forward or args.time_train:
         net = Net(args.side, args.side, args.batch, seed=args.seed)
-        x = np.random.default_rng(args.seed).uniform(0, 1, (args.batch, args.side, args.side, 2))
+        rng = np.random.default_rng(args.seed)
+        x = rng.uniform(0, 1, (args.batch, args.side, args.side, 2))
+        t = rng.normal(0, 1, (args.batch, args.side, args.side, 3))
+        reps = args.time_forward or args.time_train
+        run = (lambda: net.forward(x)) if args.time_forward else (lambda: net.train_step(x, t, 0.0))
         for _ in range(2):
-            net.forward(x)
+            run()
         t0 = time.perf_counter()
-        for _ in range(args.time_forward):
-            net.forward(x)
-        ms = (time.perf_counter() - t0) / args.time_forward * 1e3
+        for _ in range(reps):
+            run()
+        ms = (time.perf_counter() - t0) / reps * 1e3
         net.close()
-        print(json.dumps({"side": args.side, "batch": args.batch, "reps": args.time_forward,
-                          "ms_per_forward_with_host_copies": ms}), flush=True)
+        what = "forward" if args.time_forward else "train_step"
+        print(json.dumps({"side": args.side, "batch": args.batch, "reps": reps,
+                          f"ms_per_{what}_with_host_copies": ms}), flush=True)
         return 0
     if args.eval_only:
         ev = evaluate(args.eval_only, args.side, args.eval_runs, args.eval_steps, seed=args.seed + 1000)
