@@ -1797,11 +1797,12 @@ static int mixed_params(mcl_ctx *c, const UzawaParams &P, MixedParams &M)
     M.inner_rtol = k.inner_rtol > 0 ? k.inner_rtol : 3e-7;
     {
         // relaxed inner accuracy in the Uzawa loop, then one correction solve
-        // (measured at 256x256 over steps 0-19: 2.3x less solve time, Vrms
-        // within 5e-8 of converged FP64 as without it); MCL_INNER_RELAX=0
-        // switches it off
+        // (measured at 256x256 over steps 0-19 against converged FP64: relax
+        // 3e-8 gave 2.3x less solve time than none with Vrms within 8e-8;
+        // 1e-7 another 10 % with Vrms within 1.5e-7; 3e-7 8.6e-7; 1e-6
+        // 3.2e-6, past the 1e-6 bar); MCL_INNER_RELAX=0 switches it off
         const char *e = getenv("MCL_INNER_RELAX"), *m = getenv("MCL_INNER_RELAX_MAX");
-        M.relax = e ? atof(e) : 3e-8;
+        M.relax = e ? atof(e) : 1e-7;
         M.relax_max = m && atof(m) > 0 ? atof(m) : 1e-1;
     }
     M.max_inner = 1000;
