@@ -221,6 +221,14 @@ struct MixedParams {
     int use_nbr;
     int pre;                 // Jacobi sweeps per level on each leg of the V-cycle (>= 1)
     float omega;             // damped-Jacobi weight of the smoother
+    // augmented Lagrangian (al_state != null): U.L's K is K + g G W G^T with
+    // W = diag(al_w) and g = *al_state (0 or al_gamma), and the solve returns
+    // p + g W G^T d; it then sets *al_state for the next solve: al_gamma if
+    // its Uzawa count reached al_keep (g > 0) or al_start (g = 0), else 0
+    double *al_state;
+    const double *al_w;      // [nel] 1 / BAB of the plain K
+    double al_gamma;
+    int al_keep, al_start;
 };
 
 // Reduction slots in RedBuf (each slot holds MCL_MAX_WG partials).
@@ -263,6 +271,18 @@ struct PrepLevel {
     int blk0;
 };
 __global__ void k_prepare_fp32(const PrepLevel *lv, int nlev);
+// One multigrid level of the augmented-Lagrangian launches: K_al = K +
+// gamma G W G^T (W = diag(w), w_e = 1 / sum_i G_ie^2 / K_ii, the BAB of the
+// plain K), same ELL pattern, gamma read from the device.  Elements occupy workgroups [eblk0, ...) of
+// k_al_weights, rows [rblk0, ...) of k_al_rows.
+struct AlLevel {
+    LevelDev L;
+    double *val, *diag, *w;
+    const double *gamma;     // device scalar (0: K_al = K)
+    int eblk0, rblk0;
+};
+__global__ void k_al_weights(const AlLevel *lv, int nlev);
+__global__ void k_al_rows(const AlLevel *lv, int nlev);
 __global__ void k_coarse_inverse(int n, int W, const int *col, const int *ncol, const double *val, double *inv);
 __global__ void k_inject(int nc, const int *c2f, const double *fine, double *coarse);
 __global__ void k_zero_words(unsigned *p, long long n);

@@ -64,6 +64,70 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_prepare_fp32(const PrepLevel *__r
     if ((threadIdx.x & (MCL_WAVE - 1)) == 0) atomicMax(A.bound, __float_as_uint(m));
 }
 
+// Augmented Lagrangian of the mixed solve: K_al = K + gamma G W G^T with W =
+// diag(w), w_e = 1 / BAB_e of the plain K (BAB_e = sum_i G_ie^2 / K_ii,
+// Solver.c:54-64, the Uzawa preconditioner's diagonal).  G W G^T couples the
+// velocity rows of one element, so K_al keeps K's ELL pattern.  The term
+// vanishes on divergence-free velocities, so (u, p) solves K u + G p = F,
+// G^T u = 0 exactly when (u, p - gamma W G^T u) solves the augmented system;
+// its Schur complement S_al = G^T K_al^-1 G has S_al^-1 = S^-1 + gamma W^-1,
+// a narrower spectrum relative to BAB, so the Uzawa iteration needs fewer
+// steps (its inner solves somewhat more).
+
+// w_e, one thread per element of every level
+__global__ void __launch_bounds__(MCL_BLOCK) k_al_weights(const AlLevel *__restrict__ lv, int nlev)
+{
+    int lk = 0;
+    while (lk + 1 < nlev && (int)blockIdx.x >= lv[lk + 1].eblk0) lk++;
+    const AlLevel &A = lv[lk];
+    const LevelDev &L = A.L;
+    const int e = ((int)blockIdx.x - A.eblk0) * MCL_BLOCK + (int)threadIdx.x;
+    if (e >= L.nel) return;
+    double s = 0.0;
+    const int lev = L.elev[e];
+    for (int t = 0; t < 8; t++) {
+        const int r = L.gt_row[t * L.nel + e];
+        if (r < 0) break;
+        const double g = L.gtab[lev * 8 + L.gt_p[t * L.nel + e]];
+        s += g * g / L.Kdiag[r];
+    }
+    A.w[e] = s > 0.0 ? 1.0 / s : 0.0;
+}
+
+// K_al rows, one thread per row of every level: the row of K, then for each
+// element at the row's node (ascending) and each of its free dofs q the term
+// gamma w_e G_Pe G_qe in the slot of q (Kslot); diag = the diagonal slot
+__global__ void __launch_bounds__(MCL_BLOCK) k_al_rows(const AlLevel *__restrict__ lv, int nlev)
+{
+    int lk = 0;
+    while (lk + 1 < nlev && (int)blockIdx.x >= lv[lk + 1].rblk0) lk++;
+    const AlLevel &A = lv[lk];
+    const LevelDev &L = A.L;
+    const int n = L.neq;
+    const int P = ((int)blockIdx.x - A.rblk0) * MCL_BLOCK + (int)threadIdx.x;
+    if (P >= n) return;
+    for (int s = 0; s < L.WK; s++) A.val[s * n + P] = L.Kval[s * n + P];
+    const double gamma = *A.gamma;
+    if (gamma == 0.0) {
+        A.diag[P] = L.Kdiag[P];
+        return;
+    }
+    const int nd = L.veq_node[P], ip = L.veq_dof[P];
+    int sd = -1;
+    for (int k = 0; k < L.p2n_cnt[nd]; k++) {
+        const int e = L.p2n_el[k * L.nno + nd], a = L.p2n_a[k * L.nno + nd];
+        const double *g = L.gtab + L.elev[e] * 8;
+        const double c = gamma * A.w[e] * g[2 * a + ip];
+        for (int q = 0; q < 8; q++) {
+            const int s = L.Kslot[(k * 8 + q) * n + P];
+            if (s < 0) continue;
+            if (q == 2 * a + ip) sd = s;
+            A.val[s * n + P] += c * g[q];
+        }
+    }
+    A.diag[P] = sd >= 0 ? A.val[sd * n + P] : L.Kdiag[P];
+}
+
 // Dense inverse of the coarsest K by in-place Gauss-Jordan elimination (K is
 // symmetric positive definite: no pivoting).  One workgroup, n <= 64.
 __global__ void k_coarse_inverse(int n, int W, const int *__restrict__ col, const int *__restrict__ ncol,
@@ -1406,6 +1470,9 @@ __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
     const LevelDev &L = U.L;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsize = gridDim.x * blockDim.x;
     const int neq = L.neq, nel = L.nel;
+    // this solve's augmented-Lagrangian weight (read before the first wait;
+    // workgroup 0 rewrites it after the last)
+    const double al_g = P.al_state ? *P.al_state : 0.0;
     unsigned gen = __hip_atomic_load(U.gs.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     bool ok = true;
     long long spmv = 0, inner = 0;
@@ -1603,7 +1670,15 @@ __device__ __forceinline__ void uzawa_mixed(const MixedParams &P)
             else break;
         }
     }
+    if (al_g > 0.0 && ok) {
+        // the plain system's pressure p + gamma W G^T d (d was written by
+        // its owners with plain stores: a fenced barrier first)
+        ok = grid_sync(U.gs, gen);
+        if (ok)
+            for (int e = gtid; e < nel; e += gsize) U.p[e] += al_g * P.al_w[e] * gt_elem(L, e, U.d);
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (P.al_state) *P.al_state = cycle >= (al_g > 0.0 ? P.al_keep : P.al_start) ? P.al_gamma : 0.0;
         U.stats->cycles = cycle;
         U.stats->dv = dv;
         U.stats->dp = dp;

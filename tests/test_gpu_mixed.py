@@ -7,8 +7,10 @@ tolerance 1e-20, no cycle cap in practice) so that both solutions are accurate;
 the reference's own settings (absolute break at 1e-8, 200 cycles) leave a
 momentum residual of ~3e-4 on this case.  Bars: velocity and pressure agree to
 1e-5 / 1e-4 relative L2 (the Uzawa tolerance leaves differences of that order
-between two converged solves), the true momentum residual of the mixed
-solution is below 1e-9, and warm-started steps keep agreeing.
+between two converged solves; pressure without its constant mode, which the
+closed box leaves free and the mixed path's augmented-Lagrangian correction
+p + gamma W G^T u moves), the true momentum residual of the mixed solution is
+below 1e-9, and warm-started steps keep agreeing.
 """
 import math
 
@@ -43,7 +45,10 @@ def test_mixed_matches_fp64_one_solve():
     print(f"  mixed: uzawa={rb['uzawa']} inner={rb['inner']} ms={rb['solve_ms']:.2f} res={rb['res_mom']:.2e}")
     assert ra["converged"] and rb["converged"]
     assert rel_l2(b.field("D"), a.field("D")) < 1e-5
-    assert rel_l2(b.field("P"), a.field("P")) < 1e-4
+    pa, pb = a.field("P"), b.field("P")
+    ep = rel_l2(pb - pb.mean(), pa - pa.mean())
+    print(f"  |D| {rel_l2(b.field('D'), a.field('D')):.2e} |P - mean| {ep:.2e}", flush=True)
+    assert ep < 1e-4
     assert rb["res_mom"] < 1e-9
 
 
