@@ -1867,9 +1867,11 @@ static int mixed_params(mcl_ctx *c, const UzawaParams &P, MixedParams &M)
         // (measured at 256x256 over steps 0-19 against converged FP64: relax
         // 3e-8 gave 2.3x less solve time than none with Vrms within 8e-8;
         // 1e-7 another 10 % with Vrms within 1.5e-7; 3e-7 8.6e-7; 1e-6
-        // 3.2e-6, past the 1e-6 bar); MCL_INNER_RELAX=0 switches it off
+        // 3.2e-6, past the 1e-6 bar.  With the augmented velocity block in
+        // the high-count solves: 1e-7 5.0e-8, 2e-7 2.0e-7 and 5 % less
+        // time, 3e-7 3.0e-7 and 7 %); MCL_INNER_RELAX=0 switches it off
         const char *e = getenv("MCL_INNER_RELAX"), *m = getenv("MCL_INNER_RELAX_MAX");
-        M.relax = e ? atof(e) : 1e-7;
+        M.relax = e ? atof(e) : 2e-7;
         M.relax_max = m && atof(m) > 0 ? atof(m) : 1e-1;
     }
     M.max_inner = 1000;
