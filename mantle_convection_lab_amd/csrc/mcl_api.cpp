@@ -80,6 +80,11 @@ struct mcl_ctx {
     int prep_n = 0, prep_blocks = 0;
     double al_gamma = 0.0;               // > 0: the mixed solve runs on K_al (al_tab: [prep_n] levels)
     double *al_state = nullptr;          // device: the next solve's weight (al_gamma or 0)
+    // warm-start extrapolation of the mixed solve (MixedParams::ws): history
+    // and previous solution on the device; extrap_done: this step's
+    // extrapolation already ran (before the learned guess's gate)
+    double *ws = nullptr, *d_old = nullptr, *p_old = nullptr;
+    bool extrap_on = true, extrap_done = false;
     const AlLevel *al_tab = nullptr;
     int al_eblocks = 0, al_rblocks = 0;
     std::vector<Level> levels;
@@ -1008,6 +1013,18 @@ extern "C" int mcl_mesh_upload(mcl_ctx *c, const mcl_mesh_view *levels, int nlev
         c->asm_blocks = blk;
     }
     {
+        // warm-start history of the mixed solve (MCL_EXTRAPOLATE=0 turns the
+        // extrapolation off); zeroed, so a new mesh starts without history
+        Level &Fv = c->levels.back();
+        const char *e = getenv("MCL_EXTRAPOLATE");
+        c->extrap_on = !(e && atoi(e) == 0);
+        c->extrap_done = false;
+        int rc = dev_alloc(c, Fv.owned, &c->ws, 3);
+        rc |= dev_alloc(c, Fv.owned, &c->d_old, std::max(Fv.neq, 1));
+        rc |= dev_alloc(c, Fv.owned, &c->p_old, std::max(Fv.nel, 1));
+        if (rc) return MCL_E_DEVICE;
+    }
+    {
         // augmented Lagrangian of the mixed solve's levels (MCL_AL_GAMMA,
         // default 3; 0 switches it off): on meshes without hanging nodes
         // whose fixed velocities are zero (else F would need gamma G W G^T
@@ -1176,6 +1193,9 @@ extern "C" int mcl_set_field(mcl_ctx *c, int lev, int field, const double *src, 
     HIPCHK(c, hipMemcpyAsync(p, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (field == MCL_FIELD_HEAT_D) c->first_heat = false;
+    if (c->ws && L == &c->levels.back() && (field == MCL_FIELD_T || field == MCL_FIELD_D || field == MCL_FIELD_P))
+        HIPCHK(c, hipMemsetAsync(c->ws, 0, sizeof(double), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 
@@ -1892,6 +1912,10 @@ static int mixed_params(mcl_ctx *c, const UzawaParams &P, MixedParams &M)
     const char *sw = getenv("MCL_MG_SWEEPS");
     M.pre = sw && atoi(sw) > 0 ? atoi(sw) : 2;
     M.omega = 4.0f / 3.0f;
+    if (c->extrap_on && k.warm_start) {
+        M.ws = c->ws;
+        M.model_time = c->model_time;
+    }
     if (c->al_gamma > 0) {
         Level &F = c->levels.back();
         M.U.L.Kval = F.kval_al;
@@ -1928,6 +1952,21 @@ struct EventPair {
 // (captured into a batch's graph: zero-fill kernel, plain launch, no events,
 // no host synchronisation).
 enum EnqMode { ENQ_NORMAL, ENQ_NOALLOC, ENQ_GRAPH };
+
+// The mixed solve's warm start from the last two solutions (k_extrapolate),
+// once per step: before the learned guess's gate if there is one, else
+// before the solve.
+static int extrapolate(mcl_ctx *c)
+{
+    if (c->ctl.precision != MCL_PREC_MIXED || !c->ctl.warm_start || !c->extrap_on || c->extrap_done) return 0;
+    Level &L = c->levels.back();
+    const long long n = (long long)L.neq + L.nel;
+    hipLaunchKernelGGL(k_extrapolate, dim3(grid_for(c, n)), dim3(MCL_BLOCK), 0, c->stream, L.neq, L.nel, L.d, L.p,
+                       c->d_old, c->p_old, (const double *)c->ws, (const double *)c->model_time, 2.0);
+    HIPCHK(c, hipGetLastError());
+    c->extrap_done = true;
+    return 0;
+}
 
 // The device work of solve_velocity_pressure (Solver.c:3-12) on work space W,
 // up to the residual norms in W.resid; `accuracy` is the inner tolerance.
@@ -1966,6 +2005,9 @@ static int solve_enqueue(mcl_ctx *c, const SolveWork &W, EnqMode mode, EventPair
         MixedParams M;
         rc = mixed_params(c, P, M);
         if (rc) return rc;
+        rc = extrapolate(c);
+        if (rc) return rc;
+        c->extrap_done = false;
         void *margs[] = {&M};
         if (ev) HIPCHK(c, hipEventRecord(ev->a, c->stream));
         HIPCHK(c, launch_persistent(c, (const void *)k_uzawa_mixed, M.U.gs.nblocks, margs, plain));
@@ -2707,7 +2749,10 @@ extern "C" int mcl_stokes_guess(mcl_ctx *c, mcl_net *net, int *accepted, double 
                        (const double *)(c->net_rows + 2 * H), c->gp);
     HIPCHK(c, hipGetLastError());
     // gate: the initial Schur residual of each candidate (the current fields
-    // are the previous step's solution, or zero on a cold start)
+    // are the previous step's solution, extrapolated when there are two, or
+    // zero on a cold start)
+    rc = extrapolate(c);
+    if (rc) return rc;
     double rp = 0, rn = 0;
     rc = uzawa_probe(c, L.d, L.p, &rp);
     if (rc) return rc;
@@ -2717,6 +2762,9 @@ extern "C" int mcl_stokes_guess(mcl_ctx *c, mcl_net *net, int *accepted, double 
     if (acc) {
         HIPCHK(c, hipMemcpyAsync(L.d, c->gd, std::max(L.neq, 1) * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(L.p, c->gp, L.nel * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        // the fields no longer hold the last solution: no extrapolation
+        // until two solutions follow
+        HIPCHK(c, hipMemsetAsync(c->ws, 0, sizeof(double), c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     if (accepted) *accepted = acc;

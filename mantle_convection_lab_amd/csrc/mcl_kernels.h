@@ -229,6 +229,11 @@ struct MixedParams {
     const double *al_w;      // [nel] 1 / BAB of the plain K
     double al_gamma;
     int al_keep, al_start;
+    // warm-start history (null: none): ws[0] solves since the history was
+    // reset, ws[1], ws[2] the model time of the last two; the solve appends
+    // itself after its last phase (model_time: the device's model time)
+    double *ws;
+    const double *model_time;
 };
 
 // Reduction slots in RedBuf (each slot holds MCL_MAX_WG partials).
@@ -282,6 +287,10 @@ struct AlLevel {
     int eblk0, rblk0;
 };
 __global__ void k_al_weights(const AlLevel *lv, int nlev);
+// warm start from the last two solutions (MixedParams::ws): d = d1 + r (d1 -
+// d0), same for p, r = the model-time step ratio (at most rmax), then d0 = d1
+__global__ void k_extrapolate(int nv, int np, double *d, double *p, double *d_old, double *p_old, const double *ws,
+                              const double *model_time, double rmax);
 __global__ void k_al_rows(const AlLevel *lv, int nlev);
 __global__ void k_coarse_inverse(int n, int W, const int *col, const int *ncol, const double *val, double *inv);
 __global__ void k_inject(int nc, const int *c2f, const double *fine, double *coarse);
