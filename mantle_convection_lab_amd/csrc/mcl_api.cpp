@@ -83,7 +83,7 @@ struct mcl_ctx {
     // warm-start extrapolation of the mixed solve (MixedParams::ws): history
     // and previous solution on the device; extrap_done: this step's
     // extrapolation already ran (before the learned guess's gate)
-    double *ws = nullptr, *d_old = nullptr, *p_old = nullptr;
+    double *ws = nullptr, *hist = nullptr;
     bool extrap_on = true, extrap_done = false;
     const AlLevel *al_tab = nullptr;
     int al_eblocks = 0, al_rblocks = 0;
@@ -1020,8 +1020,7 @@ extern "C" int mcl_mesh_upload(mcl_ctx *c, const mcl_mesh_view *levels, int nlev
         c->extrap_on = !(e && atoi(e) == 0);
         c->extrap_done = false;
         int rc = dev_alloc(c, Fv.owned, &c->ws, 3);
-        rc |= dev_alloc(c, Fv.owned, &c->d_old, std::max(Fv.neq, 1));
-        rc |= dev_alloc(c, Fv.owned, &c->p_old, std::max(Fv.nel, 1));
+        rc |= dev_alloc(c, Fv.owned, &c->hist, std::max<size_t>((size_t)Fv.neq + Fv.nel, 1));
         if (rc) return MCL_E_DEVICE;
     }
     {
@@ -1962,7 +1961,7 @@ static int extrapolate(mcl_ctx *c)
     Level &L = c->levels.back();
     const long long n = (long long)L.neq + L.nel;
     hipLaunchKernelGGL(k_extrapolate, dim3(grid_for(c, n)), dim3(MCL_BLOCK), 0, c->stream, L.neq, L.nel, L.d, L.p,
-                       c->d_old, c->p_old, (const double *)c->ws, (const double *)c->model_time, 2.0);
+                       c->hist, (const double *)c->ws, (const double *)c->model_time, 2.0);
     HIPCHK(c, hipGetLastError());
     c->extrap_done = true;
     return 0;

@@ -287,9 +287,10 @@ struct AlLevel {
     int eblk0, rblk0;
 };
 __global__ void k_al_weights(const AlLevel *lv, int nlev);
-// warm start from the last two solutions (MixedParams::ws): d = d1 + r (d1 -
-// d0), same for p, r = the model-time step ratio (at most rmax), then d0 = d1
-__global__ void k_extrapolate(int nv, int np, double *d, double *p, double *d_old, double *p_old, const double *ws,
+// warm start from the last two solutions (MixedParams::ws): d, p extrapolated
+// linearly in model time from themselves and the previous solution h ([nv +
+// np]: d then p), step ratio at most rmax; then h = d, p
+__global__ void k_extrapolate(int nv, int np, double *d, double *p, double *h, const double *ws,
                               const double *model_time, double rmax);
 __global__ void k_al_rows(const AlLevel *lv, int nlev);
 __global__ void k_coarse_inverse(int n, int W, const int *col, const int *ncol, const double *val, double *inv);

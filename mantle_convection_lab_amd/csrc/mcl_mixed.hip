@@ -129,27 +129,30 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_al_rows(const AlLevel *__restrict
 }
 
 // Warm start of a mixed solve by linear extrapolation in model time: with
-// d1, p1 the last solution (in d, p) at time t1 and d0, p0 the one before at
-// t0 (in d_old, p_old), the new solve at time t starts from d1 + r (d1 - d0),
-// r = (t - t1) / (t1 - t0) clamped to [0, rmax]; the velocity changes smoothly
-// from step to step, so this removes most of the step's change (at 256x256,
-// Ra 1e6, step 20: initial velocity error 2.7e-3 instead of 5.1e-2).  Fewer
-// than two solutions since the history was reset: no extrapolation.  Either
-// way d1, p1 move to d_old, p_old.
+// x1 the last solution (in d, p; at time t1) and x0 the one before (in the
+// history array h; at t0), the new solve at time t starts from
+// x1 + r (x1 - x0), r = (t - t1) / (t1 - t0), as long as 0 < r <= rmax; the
+// velocity changes smoothly from step to step (at 256x256, Ra 1e6, step 21:
+// initial velocity error 4.8e-2 from x1 alone, 2.5e-3 extrapolated).  Then
+// h = x1.  Fewer than two solutions since the history was reset: no
+// extrapolation.  (Quadratic extrapolation from three solutions starts closer
+// still, 1.3e-4, but the Q1-P0 checkerboard pressure mode, which the Uzawa
+// iteration barely corrects, then grows from step to step: from step ~40 the
+// Uzawa counts oscillate with period 3 between 5 and 130.)
 __global__ void __launch_bounds__(MCL_BLOCK) k_extrapolate(int nv, int np, double *__restrict__ d,
-                                                           double *__restrict__ p, double *__restrict__ d_old,
-                                                           double *__restrict__ p_old, const double *__restrict__ ws,
+                                                           double *__restrict__ p, double *__restrict__ h,
+                                                           const double *__restrict__ ws,
                                                            const double *__restrict__ model_time, double rmax)
 {
-    double r = 0.0;
-    if (ws[0] >= 2.0 && ws[1] > ws[2]) r = fmin(fmax((*model_time - ws[1]) / (ws[1] - ws[2]), 0.0), rmax);
-    const int gs = gridDim.x * blockDim.x;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nv + np; i += gs) {
+    const double t = *model_time, t1 = ws[1], t0 = ws[2];
+    const double r = ws[0] >= 2.0 && t1 > t0 ? (t - t1) / (t1 - t0) : 0.0;
+    const bool ex = r > 0.0 && r <= rmax;
+    const int n = nv + np, gs = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gs) {
         double *x = i < nv ? d + i : p + (i - nv);
-        double *x0 = i < nv ? d_old + i : p_old + (i - nv);
         const double x1 = *x;
-        if (r > 0.0) *x = x1 + r * (x1 - *x0);
-        *x0 = x1;
+        if (ex) *x = x1 + r * (x1 - h[i]);
+        h[i] = x1;
     }
 }
 
