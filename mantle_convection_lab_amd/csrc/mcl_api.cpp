@@ -84,6 +84,7 @@ struct mcl_ctx {
     // and previous solution on the device; extrap_done: this step's
     // extrapolation already ran (before the learned guess's gate)
     double *ws = nullptr, *hist = nullptr;
+    int extrap_after = 5;                // solves after a reset before it extrapolates
     bool extrap_on = true, extrap_done = false;
     const AlLevel *al_tab = nullptr;
     int al_eblocks = 0, al_rblocks = 0;
@@ -1013,11 +1014,13 @@ extern "C" int mcl_mesh_upload(mcl_ctx *c, const mcl_mesh_view *levels, int nlev
         c->asm_blocks = blk;
     }
     {
-        // warm-start history of the mixed solve (MCL_EXTRAPOLATE=0 turns the
-        // extrapolation off); zeroed, so a new mesh starts without history
+        // warm-start history of the mixed solve (MCL_EXTRAPOLATE=n:
+        // extrapolate once n solves have followed a reset, default 5; 0
+        // turns it off); zeroed, so a new mesh starts without history
         Level &Fv = c->levels.back();
         const char *e = getenv("MCL_EXTRAPOLATE");
-        c->extrap_on = !(e && atoi(e) == 0);
+        c->extrap_after = e ? atoi(e) : 5;
+        c->extrap_on = c->extrap_after > 0;
         c->extrap_done = false;
         int rc = dev_alloc(c, Fv.owned, &c->ws, 3);
         rc |= dev_alloc(c, Fv.owned, &c->hist, std::max<size_t>((size_t)Fv.neq + Fv.nel, 1));
@@ -1961,7 +1964,7 @@ static int extrapolate(mcl_ctx *c)
     Level &L = c->levels.back();
     const long long n = (long long)L.neq + L.nel;
     hipLaunchKernelGGL(k_extrapolate, dim3(grid_for(c, n)), dim3(MCL_BLOCK), 0, c->stream, L.neq, L.nel, L.d, L.p,
-                       c->hist, (const double *)c->ws, (const double *)c->model_time, 2.0);
+                       c->hist, (const double *)c->ws, (const double *)c->model_time, c->extrap_after, 2.0);
     HIPCHK(c, hipGetLastError());
     c->extrap_done = true;
     return 0;

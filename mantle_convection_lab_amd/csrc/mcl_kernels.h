@@ -289,9 +289,10 @@ struct AlLevel {
 __global__ void k_al_weights(const AlLevel *lv, int nlev);
 // warm start from the last two solutions (MixedParams::ws): d, p extrapolated
 // linearly in model time from themselves and the previous solution h ([nv +
-// np]: d then p), step ratio at most rmax; then h = d, p
+// np]: d then p) once `after` solves (at least 2) have followed a reset of
+// the history, step ratio at most rmax; then h = d, p
 __global__ void k_extrapolate(int nv, int np, double *d, double *p, double *h, const double *ws,
-                              const double *model_time, double rmax);
+                              const double *model_time, int after, double rmax);
 __global__ void k_al_rows(const AlLevel *lv, int nlev);
 __global__ void k_coarse_inverse(int n, int W, const int *col, const int *ncol, const double *val, double *inv);
 __global__ void k_inject(int nc, const int *c2f, const double *fine, double *coarse);

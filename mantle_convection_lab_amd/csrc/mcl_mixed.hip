@@ -134,18 +134,24 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_al_rows(const AlLevel *__restrict
 // x1 + r (x1 - x0), r = (t - t1) / (t1 - t0), as long as 0 < r <= rmax; the
 // velocity changes smoothly from step to step (at 256x256, Ra 1e6, step 21:
 // initial velocity error 4.8e-2 from x1 alone, 2.5e-3 extrapolated).  Then
-// h = x1.  Fewer than two solutions since the history was reset: no
-// extrapolation.  (Quadratic extrapolation from three solutions starts closer
+// h = x1.  Only from the (after + 1)-th solve since the history was reset:
+// a cold start's first steps are a fast transient (the flow accelerates from
+// rest, dt shrinks by up to half a step to the next) where the straight line
+// through the last two solutions often overshoots (measured at 256x256,
+// Ra 1e6: steps 4 and 7 took 14 % and 40 % longer with it; on 96 seeded
+// instances of sides 32-512, Ra up to 1e7, rounds 2-4 took 0.43, 0.28 and
+// 0.26 s longer).  (Quadratic extrapolation from three solutions starts closer
 // still, 1.3e-4, but the Q1-P0 checkerboard pressure mode, which the Uzawa
 // iteration barely corrects, then grows from step to step: from step ~40 the
 // Uzawa counts oscillate with period 3 between 5 and 130.)
 __global__ void __launch_bounds__(MCL_BLOCK) k_extrapolate(int nv, int np, double *__restrict__ d,
                                                            double *__restrict__ p, double *__restrict__ h,
                                                            const double *__restrict__ ws,
-                                                           const double *__restrict__ model_time, double rmax)
+                                                           const double *__restrict__ model_time, int after,
+                                                           double rmax)
 {
     const double t = *model_time, t1 = ws[1], t0 = ws[2];
-    const double r = ws[0] >= 2.0 && t1 > t0 ? (t - t1) / (t1 - t0) : 0.0;
+    const double r = ws[0] >= (double)max(after, 2) && t1 > t0 ? (t - t1) / (t1 - t0) : 0.0;
     const bool ex = r > 0.0 && r <= rmax;
     const int n = nv + np, gs = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gs) {
