@@ -139,8 +139,8 @@ __global__ void __launch_bounds__(MCL_BLOCK) k_al_rows(const AlLevel *__restrict
 // rest, dt shrinks by up to half a step to the next) where the straight line
 // through the last two solutions often overshoots (measured at 256x256,
 // Ra 1e6: steps 4 and 7 took 14 % and 40 % longer with it; on 96 seeded
-// instances of sides 32-512, Ra up to 1e7, rounds 2-4 took 0.43, 0.28 and
-// 0.26 s longer).  (Quadratic extrapolation from three solutions starts closer
+// instances of sides 32-512, Ra up to 1e7, their steps 2-4 took 0.43, 0.28
+// and 0.26 s longer in all).  (Quadratic extrapolation from three solutions starts closer
 // still, 1.3e-4, but the Q1-P0 checkerboard pressure mode, which the Uzawa
 // iteration barely corrects, then grows from step to step: from step ~40 the
 // Uzawa counts oscillate with period 3 between 5 and 130.)
