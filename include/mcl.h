@@ -55,7 +55,9 @@ enum mcl_schur {
 /* Arithmetic of the Stokes solve. */
 enum mcl_precision {
     MCL_PREC_FP64 = 0,       /* FP64 throughout, reference operation order */
-    MCL_PREC_MIXED = 1       /* FP64 inner CG preconditioned by an FP32 multigrid V-cycle */
+    MCL_PREC_MIXED = 1       /* FP64 inner CG preconditioned by an FP32 multigrid V-cycle, on the
+                                augmented velocity block K + 3 G W G^T while Uzawa counts are high
+                                (the returned pressure is the plain system's) */
 };
 
 /* Fields that can be moved between host and device (per quadtree level). */
@@ -151,7 +153,8 @@ typedef struct mcl_control {
     int mg_cycle, mg_finest_iteration, mg_coarse_iteration;
     int schur;                 /* enum mcl_schur */
     int precision;             /* enum mcl_precision */
-    int warm_start;            /* 1: keep d and p between steps (overrides zero_P) */
+    int warm_start;            /* 1: keep d and p between steps (overrides zero_P); MCL_PREC_MIXED
+                                  extrapolates them in model time from the last two solutions */
     double inner_rtol;         /* MCL_PREC_MIXED: relative tolerance of each inner solve (0: 3e-7) */
     int exact_dots;            /* FP64 path: 1 = dot products summed in the reference's sequential
                                   order (bit-for-bit with the reference; slower), 0 = compensated
